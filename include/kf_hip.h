@@ -1,0 +1,299 @@
+/*
+ * kf_hip.h -- C ABI of the MI355X (gfx950) KinectFusion library libkf_hip.so.
+ *
+ * The library is the HIP memory space behind the reference's stage
+ * interfaces (maximsolodukhin/KinectFusion):
+ *   Pipeline              include/kinectfusion/pipeline.hpp:64-161
+ *   PyramidSource         include/kinectfusion/depth_processing.hpp:422-463
+ *   ProjectiveIcpTracker  include/kinectfusion/icp_optimizer.hpp:107-130
+ *   MarchingCubes         include/kinectfusion/marching_cubes.hpp:16-34
+ *
+ * Conventions
+ *  - Every call returns kf_status; KF_OK is 0.  No C++ exception crosses
+ *    the ABI.  kf_last_error(ctx) describes the last failure (ctx may be
+ *    NULL for failures of kf_create).
+ *  - A context is confined to one host thread and owns one HIP stream; all
+ *    device work of the context is ordered on it.  Calls that return host
+ *    data synchronise that stream; the others only enqueue work.
+ *  - Pointers documented "device" are HIP device pointers (inputs resident
+ *    in HBM); "host" pointers are ordinary host memory.
+ *  - Poses are camera_to_world, row-major 4x4 FP64.  The 3x4 float
+ *    transforms are row-major [R | t].
+ *  - Images are row-major (index y * width + x); depth is uint16 raw sensor
+ *    units; colour is packed RGBA8 (r in the low byte).
+ *  - Vertex/normal/point maps on the device are float4 (x, y, z, 0); invalid
+ *    entries hold quiet NaN (include/kinectfusion/vector.hpp:100-103).
+ *  - Dense voxels are stored x-fastest (index (z * ny + y) * nx + x,
+ *    include/kinectfusion/vector.hpp:62-68): float store = {float tsdf,
+ *    float weight}; quantized store = {int16 tsdf, uint16 weight}; bf16
+ *    store = {uint16 bf16 tsdf bits, uint16 weight}
+ *    (include/kinectfusion/volume.hpp:28-153).  Colour voxels are float4
+ *    (r, g, b, weight) (volume.hpp:169-184).
+ */
+#ifndef KF_HIP_H
+#define KF_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kf_status {
+  KF_OK = 0,
+  KF_ERR_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference */
+  KF_ERR_OUT_OF_MEMORY = 2,
+  KF_ERR_DEVICE = 3,           /* std::runtime_error from cuda::check */
+  KF_ERR_UNSUPPORTED = 4,      /* std::logic_error: not built / no device */
+  KF_ERR_STATE = 5             /* call order violated (e.g. no pyramid yet) */
+} kf_status;
+
+/* VoxelStore, ColorStore, StorageLayout (pipeline.hpp:24-26). */
+enum { KF_VOXEL_FLOAT = 0, KF_VOXEL_QUANTIZED = 1, KF_VOXEL_BF16 = 2 };
+enum { KF_COLOR_FLOAT = 0, KF_COLOR_NONE = 1 };
+enum { KF_STORAGE_DENSE = 0, KF_STORAGE_HASHED = 1 };
+/* TsdfRuleVariant (tsdf_integration.hpp:349). */
+enum { KF_RULE_CLASSIC = 0, KF_RULE_ANGLE_WEIGHTED = 1 };
+/* IcpDampingMode (icp_correspondence.hpp:58). */
+enum { KF_DAMPING_NONE = 0, KF_DAMPING_IDENTITY = 1, KF_DAMPING_DIAGONAL = 2 };
+/* How kf_process_frame seeds ICP. */
+enum { KF_INIT_PREVIOUS = 0, KF_INIT_CONSTANT_VELOCITY = 1, KF_INIT_LEARNED = 2 };
+
+/* Everything needed to compose one pipeline: PipelineConfig
+ * (pipeline.hpp:29-46) flattened, with TsdfIntegrationOptions
+ * (tsdf_integration.hpp:31-50), RaycastOptions (raycasting.hpp:66-78),
+ * DepthProcessingOptions (depth_processing.hpp:31-46) and the level-0
+ * camera.  kf_config_defaults fills the reference defaults. */
+typedef struct kf_config {
+  /* VolumeGeometry (volume.hpp:265-287) */
+  int64_t resolution[3];
+  float voxel_size;
+  float origin[3];
+  float truncation_distance;
+  int32_t voxel_store;
+  int32_t color_store;
+  int32_t storage;
+  int32_t tsdf_rule;
+  /* TsdfIntegrationOptions */
+  float depth_scale;
+  float observation_weight;
+  float max_weight;
+  float min_depth;
+  float max_depth;
+  int32_t projective_distance;
+  int32_t distance_scaled_truncation;
+  float truncation_distance_scale;
+  /* RaycastOptions */
+  float ray_min_depth;
+  float ray_max_depth;
+  float step_scale;
+  int32_t require_all_corners; /* CornerPolicy::kRequireAll */
+  int32_t cell_normals;
+  /* DepthProcessingOptions */
+  int32_t levels;
+  float max_normal_depth_jump;
+  float max_downsample_depth_jump;
+  int32_t bilateral_filter;
+  int32_t bilateral_radius;
+  float bilateral_spatial_sigma;
+  float bilateral_depth_sigma;
+  /* level-0 camera */
+  int32_t width;
+  int32_t height;
+  float fx, fy, cx, cy;
+  /* hashed volume (KF_STORAGE_HASHED) */
+  int64_t hash_buckets;
+  int64_t block_capacity;
+  /* kf_process_frame */
+  int32_t init_mode;
+  int32_t icp_iterations[4]; /* per level; reference default 10/5/4 */
+  int32_t stage_timing;      /* record HIP events per stage in kf_process_frame */
+} kf_config;
+
+/* ProjectiveIcpOptions (icp_optimizer.hpp:82-102). */
+typedef struct kf_icp_options {
+  int64_t min_correspondences;
+  float max_point_distance;
+  float min_normal_dot;
+  float min_update_translation;
+  float min_update_rotation;
+  float max_update_translation;
+  float max_update_rotation;
+  float min_system_eigenvalue;
+  float max_condition_number;
+  int32_t damping_mode;
+  float damping_lambda;
+  int32_t adaptive_damping;
+  float lambda_increase, lambda_decrease, min_lambda, max_lambda;
+} kf_icp_options;
+
+/* IcpOutcome diagnostics (icp_optimizer.hpp:43-62).  status: 0 max
+ * iterations, 1 converged, 2 too few correspondences, 3 solve failed,
+ * 4 update too large, 5 unconstrained system, -1 level skipped. */
+typedef struct kf_icp_result {
+  int32_t status;
+  int32_t trials;
+  int32_t rejected;
+  int32_t pad;
+  int64_t correspondences;
+  double mean_distance;
+  double mean_squared_residual;
+  double min_eigenvalue;
+  double condition_number;
+  double update_translation;
+  double update_rotation;
+  double lambda;
+} kf_icp_result;
+
+/* One pyramid level as ICP consumes it (PyramidLevel,
+ * depth_processing.hpp:415-418): device views valid until the next
+ * kf_pyramid_build. */
+typedef struct kf_level_view {
+  int32_t width, height;
+  float fx, fy, cx, cy;
+  const uint16_t* depth;  /* filtered (level 0) / downsampled depth */
+  const float* vertices;  /* float4 per pixel */
+  const float* normals;   /* float4 per pixel */
+} kf_level_view;
+
+/* Raycast output (SurfaceMaps, volume.hpp:208-219): device views owned by
+ * the context, valid until the next raycast into the same level slot. */
+typedef struct kf_maps_view {
+  int32_t width, height;
+  const float* points;    /* float4 */
+  const float* normals;   /* float4 */
+  const uint32_t* colors; /* RGBA8 */
+} kf_maps_view;
+
+/* A frame to integrate (DepthFrame, tsdf_integration.hpp:68-89). */
+typedef struct kf_depth_frame {
+  const uint16_t* depth;  /* device, raw depth */
+  const uint32_t* color;  /* device or NULL */
+  const float* normals;   /* device float4 or NULL (angle rule falls back) */
+  int32_t width, height;
+  float fx, fy, cx, cy;
+  float world_to_camera[12];
+} kf_depth_frame;
+
+/* Per-stage device time of the last kf_process_frame, from HIP events. */
+typedef struct kf_stage_times {
+  float pyramid_ms;
+  float raycast_ms;
+  float icp_ms;
+  float integrate_ms;
+  float total_ms;
+} kf_stage_times;
+
+/* Result of kf_process_frame (Reconstruction::process_frame,
+ * src/app/reconstruction.cpp:184-216). */
+typedef struct kf_frame_result {
+  int32_t tracked;        /* 1 when every level succeeded and it integrated */
+  int32_t failed_level;   /* -1 or the first failing level */
+  int32_t used_learned;   /* learned warm start accepted (KF_INIT_LEARNED) */
+  int32_t pad;
+  double pose[16];
+  kf_icp_result level[4];
+} kf_frame_result;
+
+typedef struct kf_context kf_context;
+typedef struct kf_mesh kf_mesh;
+
+/* ---- lifecycle: Pipeline::create (pipeline.hpp:118-121, pipeline.cpp:129-147).
+ * No silent host fallback: KF_ERR_UNSUPPORTED when no gfx950 device is
+ * usable. */
+void kf_config_defaults(kf_config* cfg);
+void kf_icp_options_defaults(kf_icp_options* opt);
+kf_status kf_create(const kf_config* cfg, kf_context** out);
+void kf_destroy(kf_context* ctx);
+const char* kf_last_error(const kf_context* ctx);
+kf_status kf_synchronize(kf_context* ctx);
+/* Fresh volume: tsdf 1, weight 0 (volume.hpp:23,107-108). */
+kf_status kf_reset(kf_context* ctx);
+
+/* ---- PyramidSource::build / level (depth_processing.hpp:432-435;
+ * depth_processing.cpp:138-163).  depth: device raw depth of the config's
+ * width x height.  Returns the level count in *levels. */
+kf_status kf_pyramid_build(kf_context* ctx, const uint16_t* depth, int32_t* levels);
+kf_status kf_pyramid_level(kf_context* ctx, int32_t level, kf_level_view* out);
+
+/* ---- Pipeline::integrate (pipeline.hpp:78-80; TsdfIntegrator,
+ * tsdf_integration.hpp:501-524; dense full sweep, tsdf_integration.hpp:356-370).
+ * counts (host, may be NULL): when non-NULL the kernel also counts the
+ * voxels fused (counts[0]) and colour voxels fused (counts[1]) and the call
+ * synchronises. */
+kf_status kf_integrate(kf_context* ctx, const kf_depth_frame* frame, int64_t* counts);
+
+/* ---- Pipeline::raycast (pipeline.hpp:82; SurfaceRaycast,
+ * raycasting.hpp:116-245).  Renders into level slot `level` with the
+ * config camera scaled by 2^-level. */
+kf_status kf_raycast(kf_context* ctx, const double* camera_to_world, int32_t level,
+                     kf_maps_view* out);
+
+/* ---- ICP (CorrespondenceSearch::match + IcpNormalEquations,
+ * icp_correspondence.hpp:104-294).  One correspondence sweep over the
+ * given device maps (float4), reduced on the device with compensated FP32
+ * sums.  sums (host): 21 upper-triangle J^T J, 6 J^T r, distance sum,
+ * residual sum of squares.  *count (host): accepted correspondences. */
+kf_status kf_icp_reduce(kf_context* ctx, const float* live_vertices, const float* live_normals,
+                        int32_t live_width, int32_t live_height, const float* model_points,
+                        const float* model_normals, int32_t model_width, int32_t model_height,
+                        const float* model_intrinsics, const float* camera, const float* model,
+                        float max_point_distance, float min_normal_dot, double* sums,
+                        int64_t* count);
+
+/* ProjectiveIcpTracker::estimate_pose in its whole-level device-loop form
+ * (icp_optimizer.cpp:82-169, cuda/icp_correspondence.cu:216-343), with the
+ * 6x6 Cholesky solve and SE(3) composition in FP64 on the device and one
+ * synchronisation at the end.  pose_inout (host): initial camera_to_world,
+ * replaced by the estimate on success. */
+kf_status kf_icp_track_maps(kf_context* ctx, const float* live_vertices, const float* live_normals,
+                            int32_t live_width, int32_t live_height, const float* model_points,
+                            const float* model_normals, int32_t model_width, int32_t model_height,
+                            const float* model_intrinsics, const double* model_pose,
+                            double* pose_inout, int32_t iterations, const kf_icp_options* opt,
+                            kf_icp_result* result);
+
+/* Coarse-to-fine tracking of the current pyramid against the volume
+ * (Reconstruction::track_pose, src/app/reconstruction.cpp:218-250): per
+ * level, raycast the model at the running pose, then the device loop.  One
+ * synchronisation.  pose_inout (host) as above; results[levels] (host). */
+kf_status kf_track(kf_context* ctx, const kf_icp_options* opt, double* pose_inout,
+                   kf_icp_result* results);
+
+/* ---- whole frame (Reconstruction::process_frame, reconstruction.cpp:184-216):
+ * pyramid, tracking seeded per cfg.init_mode, integration when tracking
+ * succeeded.  Inputs are device pointers; the running pose stays on the
+ * device.  result (host, may be NULL): when non-NULL the call synchronises
+ * and reports the frame; when NULL nothing waits on the device.
+ * The first frame of a context (or after kf_reset) integrates at the
+ * identity pose without tracking (reconstruction.cpp:143-182). */
+kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color,
+                           const kf_icp_options* opt, kf_frame_result* result);
+/* Poses of all frames processed so far (host, 16 doubles each). */
+kf_status kf_trajectory(kf_context* ctx, double* poses, int64_t capacity, int64_t* count);
+kf_status kf_last_stage_times(kf_context* ctx, kf_stage_times* out);
+
+/* ---- volume access: Pipeline::observed_voxel_count / host_view
+ * (pipeline.hpp:87-91). */
+kf_status kf_observed_voxel_count(kf_context* ctx, int64_t* count);
+kf_status kf_volume_bytes(const kf_context* ctx, size_t* voxel_bytes, size_t* color_bytes);
+kf_status kf_volume_download(kf_context* ctx, void* voxels, float* colors);
+kf_status kf_volume_upload(kf_context* ctx, const void* voxels, const float* colors);
+/* Device pointers of the dense volume (float4 colours may be NULL). */
+kf_status kf_volume_device(kf_context* ctx, void** voxels, float** colors);
+
+/* ---- MarchingCubes::extract (marching_cubes.cpp:478-507) on the device:
+ * cell classification, stream-compacted triangles and the reference's
+ * first-encounter vertex welding order. */
+kf_status kf_mesh_extract(kf_context* ctx, float min_weight, kf_mesh** out);
+kf_status kf_mesh_size(const kf_mesh* mesh, int64_t* vertices, int64_t* triangles);
+kf_status kf_mesh_copy(const kf_mesh* mesh, float* positions, float* normals, float* colors,
+                       uint32_t* triangles);
+void kf_mesh_free(kf_mesh* mesh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KF_HIP_H */

@@ -1,0 +1,705 @@
+// C ABI of libkf_hip (include/kf_hip.h): context lifecycle, buffers, and the
+// per-frame orchestration that keeps the running pose on the device.
+//
+// Reference behaviour mirrored here:
+//  * Pipeline::create / validation (src/kinectfusion/pipeline.cpp:129-147,
+//    volume.hpp:499-508, depth_processing.cpp:74-91) -- but with no host
+//    fallback: a missing gfx950 device is an error;
+//  * Reconstruction::initialize / process_frame / track_pose
+//    (src/app/reconstruction.cpp:143-250): the first frame integrates at the
+//    identity, later frames track coarse-to-fine and integrate only when
+//    every level succeeded; a failing level leaves its initial pose.
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "kf_internal.hpp"
+
+namespace kf {
+
+[[noreturn]] void fail(kf_status status, const std::string& message) {
+  throw Failure{status, message};
+}
+
+void hip_check(hipError_t err, const char* what) {
+  if (err != hipSuccess) {
+    const kf_status s = err == hipErrorOutOfMemory ? KF_ERR_OUT_OF_MEMORY : KF_ERR_DEVICE;
+    fail(s, std::string(what) + ": " + hipGetErrorString(err));
+  }
+}
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <typename F>
+kf_status guard(kf_context* ctx, F&& body) {
+  try {
+    body();
+    return KF_OK;
+  } catch (const Failure& f) {
+    if (ctx != nullptr) ctx->error = f.message;
+    else g_create_error = f.message;
+    return f.status;
+  } catch (const std::bad_alloc&) {
+    if (ctx != nullptr) ctx->error = "host allocation failed";
+    return KF_ERR_OUT_OF_MEMORY;
+  } catch (...) {
+    if (ctx != nullptr) ctx->error = "unexpected exception";
+    return KF_ERR_DEVICE;
+  }
+}
+
+void require(bool ok, const char* msg) {
+  if (!ok) fail(KF_ERR_INVALID_ARGUMENT, msg);
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  KF_HIP(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+void dfree(void* p) {
+  if (p != nullptr) (void)hipFree(p);
+}
+
+__global__ void frame_begin_kernel(IcpState* st, int init_mode) {
+  if (threadIdx.x != 0) return;
+  st->frame_failed = 0;
+  st->failed_level = -1;
+  if (init_mode == KF_INIT_CONSTANT_VELOCITY && st->frames_done >= 2) {
+    // T_k = T_{k-1} (T_{k-2}^-1 T_{k-1}) in FP64
+    double inv2[16], rel[16], out[16];
+    rigid_inverse64(st->prev2_pose, inv2);
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) {
+        double s = 0.0;
+        for (int q = 0; q < 4; ++q) s += inv2[r * 4 + q] * st->prev_pose[q * 4 + c];
+        rel[r * 4 + c] = s;
+      }
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) {
+        double s = 0.0;
+        for (int q = 0; q < 4; ++q) s += st->prev_pose[r * 4 + q] * rel[q * 4 + c];
+        out[r * 4 + c] = s;
+      }
+    for (int i = 0; i < 16; ++i) st->pose[i] = out[i];
+  } else {
+    for (int i = 0; i < 16; ++i) st->pose[i] = st->prev_pose[i];
+  }
+}
+
+__global__ void frame_end_kernel(IcpState* st, double* trajectory, long long slot) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) {
+    trajectory[slot * 16 + i] = st->pose[i];
+    st->prev2_pose[i] = st->prev_pose[i];
+    st->prev_pose[i] = st->pose[i];
+  }
+  st->frames_done += 1;
+}
+
+__global__ void set_identity_kernel(IcpState* st) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i < 16; ++i) {
+    const double v = (i % 5 == 0) ? 1.0 : 0.0;
+    st->pose[i] = v;
+    st->prev_pose[i] = v;
+    st->prev2_pose[i] = v;
+  }
+  st->frame_failed = 0;
+  st->failed_level = -1;
+  st->frames_done = 0;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    st->results[l] = kf_icp_result{};
+    st->results[l].status = -1;
+  }
+}
+
+template <int Kind>
+__global__ void count_observed_kernel(const typename VoxelIO<Kind>::Raw* v, size_t n,
+                                      unsigned long long* out) {
+  unsigned long long c = 0;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    float t, w;
+    bool obs, fin;
+    VoxelIO<Kind>::unpack(v[i], t, w, obs, fin);
+    if (w > 0.0F) ++c;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, c);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p) {
+  if (threadIdx.x == 0) *p = 0ULL;
+}
+
+void validate(const kf_config& c) {
+  // VolumeGeometry (volume.hpp:499-508)
+  require(c.resolution[0] > 1 && c.resolution[1] > 1 && c.resolution[2] > 1,
+          "Volume resolution must be at least 2 along each axis");
+  require(c.resolution[0] <= 4096 && c.resolution[1] <= 4096 && c.resolution[2] <= 4096,
+          "Volume resolution above 4096 per axis is not supported");
+  require(c.voxel_size > 0.0F, "Voxel size must be positive");
+  require(c.truncation_distance > 0.0F, "Truncation distance must be positive");
+  require(std::isfinite(c.origin[0]) && std::isfinite(c.origin[1]) && std::isfinite(c.origin[2]),
+          "Volume origin must be finite");
+  require(c.voxel_store >= 0 && c.voxel_store <= 2, "Unknown voxel store");
+  require(c.color_store == KF_COLOR_FLOAT || c.color_store == KF_COLOR_NONE, "Unknown color store");
+  require(c.storage == KF_STORAGE_DENSE, "Only the dense storage layout is built into this context");
+  require(c.tsdf_rule == KF_RULE_CLASSIC || c.tsdf_rule == KF_RULE_ANGLE_WEIGHTED, "Unknown TSDF rule");
+  // TsdfIntegrator::validated (tsdf_integration.cpp) and depth options
+  require(c.depth_scale > 0.0F, "Depth scale must be positive");
+  require(c.min_depth >= 0.0F && c.max_depth > c.min_depth, "Depth range is invalid");
+  require(c.max_weight > 0.0F, "Max weight must be positive");
+  require(c.levels >= 1 && c.levels <= kMaxLevels, "Pyramid levels must be in [1, 4]");
+  require(c.max_normal_depth_jump >= 0.0F, "Normal depth jump threshold must be non-negative");
+  require(c.max_downsample_depth_jump >= 0.0F, "Downsample depth jump threshold must be non-negative");
+  require(c.bilateral_radius >= 0 && c.bilateral_radius <= 8, "Bilateral radius must be in [0, 8]");
+  require(c.bilateral_spatial_sigma > 0.0F, "Bilateral spatial sigma must be positive");
+  require(c.bilateral_depth_sigma > 0.0F, "Bilateral depth sigma must be positive");
+  // RaycastOptions (raycasting.cpp:15-29)
+  require(c.ray_min_depth >= 0.0F && c.ray_max_depth > c.ray_min_depth, "Raycast depth range is invalid");
+  require(c.step_scale > 0.0F, "Raycast step scale must be positive");
+  require((c.ray_max_depth - c.ray_min_depth) / (c.voxel_size * c.step_scale) <= 1.0e6F,
+          "Raycast march would exceed 1e6 steps per ray");
+  require(c.width > 0 && c.height > 0 && c.width <= 8192 && c.height <= 8192, "Camera size is invalid");
+  require(c.fx > 0.0F && c.fy > 0.0F, "Focal lengths must be positive");
+  require(c.init_mode >= KF_INIT_PREVIOUS && c.init_mode <= KF_INIT_LEARNED, "Unknown init mode");
+  for (int l = 0; l < c.levels; ++l) require(c.icp_iterations[l] >= 1, "ICP iterations must be >= 1");
+}
+
+void upload_pose(kf_context* ctx, const double* pose) {
+  KF_HIP(hipMemcpyAsync(ctx->icp->pose, pose, 16 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+}
+
+void fetch_state(kf_context* ctx) {
+  KF_HIP(hipMemcpyAsync(ctx->icp_host, ctx->icp, sizeof(IcpState), hipMemcpyDeviceToHost, ctx->stream));
+  KF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+void build_pyramid(kf_context* ctx, const uint16_t* depth) {
+  launch_bilateral(ctx, depth);
+  for (int l = 0; l < ctx->cfg.levels; ++l) launch_level_maps(ctx, l);
+  ctx->levels_built = ctx->cfg.levels;
+}
+
+kf_depth_frame frame_from_pyramid(kf_context* ctx, const uint16_t* depth, const uint32_t* color) {
+  kf_depth_frame f{};
+  f.depth = depth;
+  f.color = color;
+  f.normals = reinterpret_cast<const float*>(ctx->lv[0].norm);
+  f.width = ctx->cfg.width;
+  f.height = ctx->cfg.height;
+  f.fx = ctx->cfg.fx;
+  f.fy = ctx->cfg.fy;
+  f.cx = ctx->cfg.cx;
+  f.cy = ctx->cfg.cy;
+  const float id[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+  std::memcpy(f.world_to_camera, id, sizeof(id));
+  return f;
+}
+
+void record(kf_context* ctx, int slot) {
+  if (ctx->cfg.stage_timing) KF_HIP(hipEventRecord(ctx->ev[slot], ctx->stream));
+}
+
+// Coarse-to-fine tracking from the device pose (track_pose).
+void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing) {
+  const kf_config& c = ctx->cfg;
+  int slot = 2;
+  for (int l = c.levels - 1; l >= 0; --l) {
+    LevelBuffers& L = ctx->lv[l];
+    launch_icp_begin_level(ctx, l);
+    launch_raycast(ctx, l, nullptr, nullptr, true);
+    if (timing) record(ctx, slot++);
+    launch_icp_level(ctx, L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k,
+                     c.icp_iterations[l], p, l);
+    if (timing) record(ctx, slot++);
+  }
+}
+
+void copy_results(const kf_context* ctx, kf_icp_result* results) {
+  for (int l = 0; l < ctx->cfg.levels; ++l) results[l] = ctx->icp_host->results[l];
+}
+
+}  // namespace
+}  // namespace kf
+
+using namespace kf;
+
+extern "C" {
+
+void kf_config_defaults(kf_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  // src/app/app_options.hpp:22-28 and the library option defaults
+  c->resolution[0] = c->resolution[1] = c->resolution[2] = 512;
+  c->voxel_size = 0.01F;
+  c->origin[0] = c->origin[1] = -0.5F * 512 * 0.01F;
+  c->origin[2] = -2.56F;
+  c->truncation_distance = 0.05F;
+  c->voxel_store = KF_VOXEL_FLOAT;
+  c->color_store = KF_COLOR_FLOAT;
+  c->storage = KF_STORAGE_DENSE;
+  c->tsdf_rule = KF_RULE_ANGLE_WEIGHTED;
+  c->depth_scale = 5000.0F;
+  c->observation_weight = 1.0F;
+  c->max_weight = 196.0F;
+  c->min_depth = 0.4F;
+  c->max_depth = 8.0F;
+  c->projective_distance = 1;
+  c->distance_scaled_truncation = 0;
+  c->truncation_distance_scale = 0.01F;
+  c->ray_min_depth = 0.4F;
+  c->ray_max_depth = 8.0F;
+  c->step_scale = 1.0F;
+  c->require_all_corners = 0;
+  c->cell_normals = 0;
+  c->levels = 3;
+  c->max_normal_depth_jump = 0.1F;
+  c->max_downsample_depth_jump = 0.1F;
+  c->bilateral_filter = 1;
+  c->bilateral_radius = 2;
+  c->bilateral_spatial_sigma = 2.0F;
+  c->bilateral_depth_sigma = 0.08F;
+  c->width = 640;
+  c->height = 480;
+  c->fx = 525.0F;
+  c->fy = 525.0F;
+  c->cx = 319.5F;
+  c->cy = 239.5F;
+  c->hash_buckets = 1 << 20;
+  c->block_capacity = 500000;
+  c->init_mode = KF_INIT_PREVIOUS;
+  c->icp_iterations[0] = 10;
+  c->icp_iterations[1] = 5;
+  c->icp_iterations[2] = 4;
+  c->icp_iterations[3] = 4;
+  c->stage_timing = 0;
+}
+
+void kf_icp_options_defaults(kf_icp_options* o) {
+  std::memset(o, 0, sizeof(*o));
+  o->min_correspondences = 64;       // icp_optimizer.hpp:14-20
+  o->max_point_distance = 0.05F;     // icp_correspondence.hpp:25-34
+  o->min_normal_dot = 0.9659258F;
+  o->min_update_translation = 1.0e-5F;
+  o->min_update_rotation = 1.0e-5F;
+  o->max_update_translation = 0.15F;
+  o->max_update_rotation = 0.35F;
+  o->min_system_eigenvalue = 1.0e-6F;
+  o->max_condition_number = 1.0e6F;
+  o->damping_mode = KF_DAMPING_NONE;
+  o->damping_lambda = 1.0e-3F;
+  o->adaptive_damping = 0;
+  o->lambda_increase = 10.0F;
+  o->lambda_decrease = 0.1F;
+  o->min_lambda = 1.0e-9F;
+  o->max_lambda = 1.0e6F;
+}
+
+kf_status kf_create(const kf_config* cfg, kf_context** out) {
+  if (out == nullptr || cfg == nullptr) {
+    g_create_error = "kf_create: null argument";
+    return KF_ERR_INVALID_ARGUMENT;
+  }
+  *out = nullptr;
+  kf_context* ctx = nullptr;
+  const kf_status s = guard(nullptr, [&] {
+    validate(*cfg);
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+      fail(KF_ERR_UNSUPPORTED, "no HIP device is available; this library has no host fallback");
+    int dev = 0;
+    KF_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop{};
+    KF_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      fail(KF_ERR_UNSUPPORTED, std::string("device is ") + prop.gcnArchName + "; built for gfx950 only");
+    ctx = new kf_context();
+    ctx->cfg = *cfg;
+    ctx->device = dev;
+    const kf_config& c = ctx->cfg;
+    KF_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    ctx->grid = Grid{static_cast<int>(c.resolution[0]), static_cast<int>(c.resolution[1]),
+                     static_cast<int>(c.resolution[2]), c.voxel_size, c.origin[0], c.origin[1],
+                     c.origin[2], c.truncation_distance};
+    ctx->n_voxels = static_cast<size_t>(c.resolution[0]) * c.resolution[1] * c.resolution[2];
+    ctx->voxels = dalloc<uint8_t>(ctx->n_voxels * voxel_bytes(c.voxel_store));
+    if (c.color_store == KF_COLOR_FLOAT) ctx->colors = dalloc<float4>(ctx->n_voxels);
+    int w = c.width, h = c.height;
+    for (int l = 0; l < c.levels; ++l) {
+      LevelBuffers& L = ctx->lv[l];
+      L.w = w;
+      L.h = h;
+      const float s2 = 1.0F / static_cast<float>(1U << l);  // CameraIntrinsics::scaled (rgbd.hpp:66-70)
+      L.k = Intr{c.fx * s2, c.fy * s2, c.cx * s2, c.cy * s2};
+      const size_t n = static_cast<size_t>(w) * h;
+      L.depth = dalloc<uint16_t>(n);
+      L.vert = dalloc<float4>(n);
+      L.norm = dalloc<float4>(n);
+      L.model_pts = dalloc<float4>(n);
+      L.model_nrm = dalloc<float4>(n);
+      L.model_col = dalloc<uint32_t>(n);
+      w /= 2;
+      h /= 2;
+      require(w > 0 || l + 1 == c.levels, "Too many pyramid levels for the camera size");
+    }
+    const size_t npx = static_cast<size_t>(c.width) * c.height;
+    ctx->pix = dalloc<float4>(npx);
+    ctx->tile_max = dalloc<float>(((c.width + kTile - 1) / kTile) * ((c.height + kTile - 1) / kTile));
+    const size_t nb = static_cast<size_t>((c.resolution[0] + kBrickX - 1) / kBrickX) *
+                      ((c.resolution[1] + kBrickY - 1) / kBrickY) *
+                      ((c.resolution[2] + kBrickZ - 1) / kBrickZ);
+    ctx->bricks = dalloc<uint32_t>(nb);
+    ctx->counters = dalloc<uint32_t>(4);
+    ctx->fused = dalloc<unsigned long long>(2);
+    ctx->partials = dalloc<float>(512 * 32);
+    ctx->ticket = dalloc<unsigned int>(1);
+    KF_HIP(hipMemsetAsync(ctx->ticket, 0, sizeof(unsigned int), ctx->stream));
+    ctx->icp = dalloc<IcpState>(1);
+    KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
+    KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));
+    ctx->traj_capacity = 4096;
+    ctx->trajectory = dalloc<double>(ctx->traj_capacity * 16);
+    for (auto& e : ctx->ev) KF_HIP(hipEventCreate(&e));
+    launch_fill_voxels(ctx);
+    hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+  if (s != KF_OK) {
+    if (ctx != nullptr) {
+      g_create_error = ctx->error;
+      kf_destroy(ctx);
+    }
+    return s;
+  }
+  *out = ctx;
+  return KF_OK;
+}
+
+void kf_destroy(kf_context* ctx) {
+  if (ctx == nullptr) return;
+  if (ctx->stream != nullptr) (void)hipStreamSynchronize(ctx->stream);
+  dfree(ctx->voxels);
+  dfree(ctx->colors);
+  for (auto& L : ctx->lv) {
+    dfree(L.depth);
+    dfree(L.vert);
+    dfree(L.norm);
+    dfree(L.model_pts);
+    dfree(L.model_nrm);
+    dfree(L.model_col);
+  }
+  dfree(ctx->pix);
+  dfree(ctx->tile_max);
+  dfree(ctx->bricks);
+  dfree(ctx->counters);
+  dfree(ctx->fused);
+  dfree(ctx->partials);
+  dfree(ctx->ticket);
+  dfree(ctx->icp);
+  dfree(ctx->trajectory);
+  if (ctx->icp_host != nullptr) (void)hipHostFree(ctx->icp_host);
+  for (auto& e : ctx->ev)
+    if (e != nullptr) (void)hipEventDestroy(e);
+  if (ctx->stream != nullptr) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* kf_last_error(const kf_context* ctx) {
+  return ctx != nullptr ? ctx->error.c_str() : g_create_error.c_str();
+}
+
+kf_status kf_synchronize(kf_context* ctx) {
+  return guard(ctx, [&] { KF_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+kf_status kf_reset(kf_context* ctx) {
+  return guard(ctx, [&] {
+    launch_fill_voxels(ctx);
+    hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
+    KF_HIP(hipGetLastError());
+    ctx->frames = 0;
+    ctx->levels_built = 0;
+  });
+}
+
+kf_status kf_pyramid_build(kf_context* ctx, const uint16_t* depth, int32_t* levels) {
+  return guard(ctx, [&] {
+    require(depth != nullptr, "kf_pyramid_build: null depth");
+    build_pyramid(ctx, depth);
+    if (levels != nullptr) *levels = ctx->levels_built;
+  });
+}
+
+kf_status kf_pyramid_level(kf_context* ctx, int32_t level, kf_level_view* out) {
+  return guard(ctx, [&] {
+    if (level < 0 || level >= ctx->levels_built) fail(KF_ERR_STATE, "kf_pyramid_level: level not built");
+    const LevelBuffers& L = ctx->lv[level];
+    out->width = L.w;
+    out->height = L.h;
+    out->fx = L.k.fx;
+    out->fy = L.k.fy;
+    out->cx = L.k.cx;
+    out->cy = L.k.cy;
+    out->depth = L.depth;
+    out->vertices = reinterpret_cast<const float*>(L.vert);
+    out->normals = reinterpret_cast<const float*>(L.norm);
+  });
+}
+
+kf_status kf_integrate(kf_context* ctx, const kf_depth_frame* frame, int64_t* counts) {
+  return guard(ctx, [&] {
+    require(frame != nullptr && frame->depth != nullptr, "kf_integrate: null frame or depth");
+    require(frame->width > 0 && frame->height > 0, "kf_integrate: empty frame");
+    require(frame->fx > 0.0F && frame->fy > 0.0F, "kf_integrate: invalid intrinsics");
+    launch_integrate(ctx, *frame, counts != nullptr, false);
+    if (counts != nullptr) {
+      unsigned long long host[2];
+      KF_HIP(hipMemcpyAsync(host, ctx->fused, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+      KF_HIP(hipStreamSynchronize(ctx->stream));
+      counts[0] = static_cast<int64_t>(host[0]);
+      counts[1] = static_cast<int64_t>(host[1]);
+    }
+  });
+}
+
+kf_status kf_raycast(kf_context* ctx, const double* camera_to_world, int32_t level, kf_maps_view* out) {
+  return guard(ctx, [&] {
+    require(camera_to_world != nullptr, "kf_raycast: null pose");
+    require(level >= 0 && level < ctx->cfg.levels, "kf_raycast: level out of range");
+    const Rigid pose = rigid_from_pose64(camera_to_world);
+    launch_raycast(ctx, level, nullptr, &pose, false);
+    const LevelBuffers& L = ctx->lv[level];
+    if (out != nullptr) {
+      out->width = L.w;
+      out->height = L.h;
+      out->points = reinterpret_cast<const float*>(L.model_pts);
+      out->normals = reinterpret_cast<const float*>(L.model_nrm);
+      out->colors = L.model_col;
+    }
+  });
+}
+
+kf_status kf_icp_reduce(kf_context* ctx, const float* lv, const float* ln, int32_t lw, int32_t lh,
+                        const float* mv, const float* mn, int32_t mw, int32_t mh, const float* k,
+                        const float* camera, const float* model, float max_dist, float min_dot,
+                        double* sums, int64_t* count) {
+  return guard(ctx, [&] {
+    require(lv && ln && mv && mn && k && camera && model && sums && count, "kf_icp_reduce: null argument");
+    require(lw > 0 && lh > 0 && mw > 0 && mh > 0, "kf_icp_reduce: empty maps");
+    launch_icp_reduce_once(ctx, reinterpret_cast<const float4*>(lv), reinterpret_cast<const float4*>(ln),
+                           lw, lh, reinterpret_cast<const float4*>(mv),
+                           reinterpret_cast<const float4*>(mn), mw, mh, Intr{k[0], k[1], k[2], k[3]},
+                           camera, model, max_dist, min_dot);
+    fetch_state(ctx);
+    for (int i = 0; i < 29; ++i) sums[i] = static_cast<double>(ctx->icp_host->sums32[i]);
+    *count = static_cast<int64_t>(__builtin_bit_cast(uint32_t, ctx->icp_host->sums32[29]));
+  });
+}
+
+kf_status kf_icp_track_maps(kf_context* ctx, const float* lv, const float* ln, int32_t lw, int32_t lh,
+                            const float* mv, const float* mn, int32_t mw, int32_t mh, const float* k,
+                            const double* model_pose, double* pose_inout, int32_t iterations,
+                            const kf_icp_options* opt, kf_icp_result* result) {
+  return guard(ctx, [&] {
+    require(lv && ln && mv && mn && k && model_pose && pose_inout && opt, "kf_icp_track_maps: null argument");
+    require(lw > 0 && lh > 0 && mw > 0 && mh > 0, "kf_icp_track_maps: empty maps");
+    require(iterations >= 1, "kf_icp_track_maps: iterations must be >= 1");
+    const DeviceIcpParams p = to_device_params(*opt);
+    // running pose = model pose for the level setup, then the initial pose
+    KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
+    upload_pose(ctx, model_pose);
+    launch_icp_begin_level(ctx, 0);
+    upload_pose(ctx, pose_inout);
+    KF_HIP(hipMemcpyAsync(ctx->icp->level_pose0, pose_inout, 16 * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    KF_HIP(hipMemcpyAsync(ctx->icp->best_pose, pose_inout, 16 * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    launch_icp_level(ctx, reinterpret_cast<const float4*>(lv), reinterpret_cast<const float4*>(ln), lw, lh,
+                     reinterpret_cast<const float4*>(mv), reinterpret_cast<const float4*>(mn), mw, mh,
+                     Intr{k[0], k[1], k[2], k[3]}, iterations, p, 0);
+    fetch_state(ctx);
+    const kf_icp_result& r = ctx->icp_host->results[0];
+    if (result != nullptr) *result = r;
+    if (r.status == 0 || r.status == 1) std::memcpy(pose_inout, ctx->icp_host->pose, 16 * sizeof(double));
+    // leave the context's tracking state untouched by this standalone call
+    KF_HIP(hipMemcpyAsync(ctx->icp->pose, ctx->icp->prev_pose, 16 * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->icp_host->frame_failed = 0;
+    KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
+  });
+}
+
+kf_status kf_track(kf_context* ctx, const kf_icp_options* opt, double* pose_inout, kf_icp_result* results) {
+  return guard(ctx, [&] {
+    require(opt != nullptr && pose_inout != nullptr, "kf_track: null argument");
+    if (ctx->levels_built == 0) fail(KF_ERR_STATE, "kf_track: build the pyramid first");
+    const DeviceIcpParams p = to_device_params(*opt);
+    upload_pose(ctx, pose_inout);
+    KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
+    track_levels(ctx, p, false);
+    fetch_state(ctx);
+    std::memcpy(pose_inout, ctx->icp_host->pose, 16 * sizeof(double));
+    if (results != nullptr) copy_results(ctx, results);
+    KF_HIP(hipMemcpyAsync(ctx->icp->pose, ctx->icp->prev_pose, 16 * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
+  });
+}
+
+kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color,
+                           const kf_icp_options* opt, kf_frame_result* result) {
+  return guard(ctx, [&] {
+    require(depth != nullptr && opt != nullptr, "kf_process_frame: null argument");
+    if (ctx->frames >= ctx->traj_capacity) {
+      double* bigger = dalloc<double>(ctx->traj_capacity * 2 * 16);
+      KF_HIP(hipMemcpyAsync(bigger, ctx->trajectory, ctx->traj_capacity * 16 * sizeof(double),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+      KF_HIP(hipStreamSynchronize(ctx->stream));
+      dfree(ctx->trajectory);
+      ctx->trajectory = bigger;
+      ctx->traj_capacity *= 2;
+    }
+    const bool timing = ctx->cfg.stage_timing != 0;
+    record(ctx, 0);
+    build_pyramid(ctx, depth);
+    record(ctx, 1);
+    const kf_depth_frame frame = frame_from_pyramid(ctx, depth, color);
+    if (ctx->frames == 0) {
+      // initialize(): integrate the first frame at the identity
+      hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
+                         static_cast<int>(KF_INIT_PREVIOUS));
+      for (int s = 2; s < 2 + 2 * ctx->cfg.levels && timing; ++s) record(ctx, s);
+      launch_integrate(ctx, frame, false, true);
+    } else {
+      hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
+                         ctx->cfg.init_mode == KF_INIT_CONSTANT_VELOCITY ? KF_INIT_CONSTANT_VELOCITY
+                                                                       : KF_INIT_PREVIOUS);
+      const DeviceIcpParams p = to_device_params(*opt);
+      track_levels(ctx, p, timing);
+      launch_integrate(ctx, frame, false, true);
+    }
+    hipLaunchKernelGGL(frame_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->trajectory,
+                       static_cast<long long>(ctx->frames));
+    KF_HIP(hipGetLastError());
+    if (timing) KF_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
+    ctx->frames += 1;
+    ctx->have_times = timing;
+    if (result != nullptr) {
+      fetch_state(ctx);
+      std::memset(result, 0, sizeof(*result));
+      result->failed_level = ctx->icp_host->frame_failed ? ctx->icp_host->failed_level : -1;
+      result->tracked = ctx->icp_host->frame_failed ? 0 : 1;
+      std::memcpy(result->pose, ctx->icp_host->pose, sizeof(result->pose));
+      if (ctx->frames > 1) copy_results(ctx, result->level);
+    }
+  });
+}
+
+kf_status kf_trajectory(kf_context* ctx, double* poses, int64_t capacity, int64_t* count) {
+  return guard(ctx, [&] {
+    const int64_t n = std::min<int64_t>(capacity, ctx->frames);
+    if (n > 0)
+      KF_HIP(hipMemcpyAsync(poses, ctx->trajectory, n * 16 * sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    if (count != nullptr) *count = ctx->frames;
+  });
+}
+
+kf_status kf_last_stage_times(kf_context* ctx, kf_stage_times* out) {
+  return guard(ctx, [&] {
+    if (!ctx->have_times) fail(KF_ERR_STATE, "stage timing is off or no frame was processed");
+    KF_HIP(hipEventSynchronize(ctx->ev[11]));
+    auto ms = [&](int a, int b) {
+      float t = 0.0F;
+      KF_HIP(hipEventElapsedTime(&t, ctx->ev[a], ctx->ev[b]));
+      return t;
+    };
+    kf_stage_times t{};
+    t.pyramid_ms = ms(0, 1);
+    // events 2.. alternate: after raycast, after icp (per level); the first
+    // raycast starts at event 1
+    int prev = 1;
+    for (int l = 0; l < ctx->cfg.levels; ++l) {
+      t.raycast_ms += ms(prev, 2 + 2 * l);
+      t.icp_ms += ms(2 + 2 * l, 3 + 2 * l);
+      prev = 3 + 2 * l;
+    }
+    t.integrate_ms = ms(prev, 11);
+    t.total_ms = ms(0, 11);
+    *out = t;
+  });
+}
+
+kf_status kf_observed_voxel_count(kf_context* ctx, int64_t* count) {
+  return guard(ctx, [&] {
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
+    const int grid = 2048;
+    switch (ctx->cfg.voxel_store) {
+      case KF_VOXEL_QUANTIZED:
+        hipLaunchKernelGGL(count_observed_kernel<KF_VOXEL_QUANTIZED>, dim3(grid), dim3(256), 0, ctx->stream,
+                           static_cast<const uint32_t*>(ctx->voxels), ctx->n_voxels, ctx->fused);
+        break;
+      case KF_VOXEL_BF16:
+        hipLaunchKernelGGL(count_observed_kernel<KF_VOXEL_BF16>, dim3(grid), dim3(256), 0, ctx->stream,
+                           static_cast<const uint32_t*>(ctx->voxels), ctx->n_voxels, ctx->fused);
+        break;
+      default:
+        hipLaunchKernelGGL(count_observed_kernel<KF_VOXEL_FLOAT>, dim3(grid), dim3(256), 0, ctx->stream,
+                           static_cast<const float2*>(ctx->voxels), ctx->n_voxels, ctx->fused);
+        break;
+    }
+    KF_HIP(hipGetLastError());
+    unsigned long long host = 0;
+    KF_HIP(hipMemcpyAsync(&host, ctx->fused, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    *count = static_cast<int64_t>(host);
+  });
+}
+
+kf_status kf_volume_bytes(const kf_context* ctx, size_t* voxel_bytes_out, size_t* color_bytes) {
+  if (ctx == nullptr) return KF_ERR_INVALID_ARGUMENT;
+  if (voxel_bytes_out != nullptr) *voxel_bytes_out = ctx->n_voxels * voxel_bytes(ctx->cfg.voxel_store);
+  if (color_bytes != nullptr) *color_bytes = ctx->colors != nullptr ? ctx->n_voxels * sizeof(float4) : 0;
+  return KF_OK;
+}
+
+kf_status kf_volume_download(kf_context* ctx, void* voxels, float* colors) {
+  return guard(ctx, [&] {
+    if (voxels != nullptr)
+      KF_HIP(hipMemcpyAsync(voxels, ctx->voxels, ctx->n_voxels * voxel_bytes(ctx->cfg.voxel_store),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    if (colors != nullptr && ctx->colors != nullptr)
+      KF_HIP(hipMemcpyAsync(colors, ctx->colors, ctx->n_voxels * sizeof(float4), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+kf_status kf_volume_upload(kf_context* ctx, const void* voxels, const float* colors) {
+  return guard(ctx, [&] {
+    if (voxels != nullptr)
+      KF_HIP(hipMemcpyAsync(ctx->voxels, voxels, ctx->n_voxels * voxel_bytes(ctx->cfg.voxel_store),
+                            hipMemcpyHostToDevice, ctx->stream));
+    if (colors != nullptr && ctx->colors != nullptr)
+      KF_HIP(hipMemcpyAsync(ctx->colors, colors, ctx->n_voxels * sizeof(float4), hipMemcpyHostToDevice,
+                            ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+kf_status kf_volume_device(kf_context* ctx, void** voxels, float** colors) {
+  if (ctx == nullptr) return KF_ERR_INVALID_ARGUMENT;
+  if (voxels != nullptr) *voxels = ctx->voxels;
+  if (colors != nullptr) *colors = reinterpret_cast<float*>(ctx->colors);
+  return KF_OK;
+}
+
+}  // extern "C"

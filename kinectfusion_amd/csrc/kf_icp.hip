@@ -1,0 +1,528 @@
+// Projective point-to-plane ICP on the device.
+//
+// Reference: CorrespondenceSearch::match, IcpCorrespondence::from_match and
+// IcpNormalEquations::accumulate (include/kinectfusion/icp_correspondence.hpp:
+// 44-294); the whole-level device loop (src/kinectfusion/cuda/
+// icp_correspondence.cu:216-343) with its host epilogue
+// (src/kinectfusion/icp_optimizer.cpp:82-169).
+//
+// One kernel per Gauss-Newton / LM trial:
+//  * every thread matches its pixels and keeps the 29 per-correspondence
+//    terms (21 J^T J, 6 J^T r, distance, r^2) in FP32 registers;
+//  * a wave reduces them with an xor butterfly, the four waves of a
+//    workgroup combine in a fixed order through LDS, and each workgroup
+//    writes one partial row -- no float atomics, so the sums do not depend
+//    on scheduling;
+//  * the last workgroup to finish (ticket counter) adds the partial rows in
+//    block order with Kahan-compensated FP32, converts to FP64 and runs the
+//    trial: FP64 pivoted LDL^T solve, caps, Rodrigues increment and pose
+//    composition in FP64, convergence and failure latches.
+// Once a level's status latches (converged or failed) the remaining trial
+// kernels return at once, so a converged level costs no further sweeps.  The
+// stability spectrum (FP64 cyclic Jacobi) is checked once per level in
+// icp_finish_level_kernel, as in the reference's device loop.
+#include "kf_internal.hpp"
+
+namespace kf {
+namespace {
+
+constexpr int kSums = 29;
+constexpr int kPartialStride = 32;
+constexpr int kMaxBlocks = 512;
+
+struct Maps {
+  const float4* __restrict__ lv;
+  const float4* __restrict__ ln;
+  const float4* __restrict__ mv;
+  const float4* __restrict__ mn;
+  int lw, lh, mw, mh;
+  Intr k;
+};
+
+// match() for one live pixel; returns false when rejected.
+__device__ inline bool match_pixel(const Maps& m, const Rigid& cam, const Rigid& mod, float max_dist,
+                                   float min_dot, int x, int y, float* J, float& res, float& dist) {
+  const size_t li = static_cast<size_t>(y) * m.lw + x;
+  const V3 lv = ld3(m.lv[li]);
+  const V3 ln = ld3(m.ln[li]);
+  if (!finite3(lv) || !finite3(ln)) return false;
+  const V3 src = xform(cam, lv);
+  const V3 sim = xform(mod, src);
+  if (!finite3(sim) || sim.z <= 0.0F) return false;
+  float u, v;
+  project(m.k, sim, u, v);
+  int col, row;
+  if (!round_index(u, m.mw, col) || !round_index(v, m.mh, row)) return false;
+  const size_t mi = static_cast<size_t>(row) * m.mw + col;
+  const V3 mv = ld3(m.mv[mi]);
+  const V3 mn = ld3(m.mn[mi]);
+  if (!finite3(mv) || !finite3(mn)) return false;
+  const V3 sn = normalize3(rot(cam, ln));
+  dist = norm3(sub3(src, mv));
+  if (dist > max_dist) return false;
+  if (dot3(sn, mn) < min_dot) return false;
+  const V3 r3 = cross3(src, mn);
+  J[0] = r3.x;
+  J[1] = r3.y;
+  J[2] = r3.z;
+  J[3] = mn.x;
+  J[4] = mn.y;
+  J[5] = mn.z;
+  res = dot3(mn, sub3(mv, src));
+  return true;
+}
+
+// Per-workgroup sweep: sums of this block's pixels into partial[blockIdx.x].
+__device__ inline void sweep_block(const Maps& m, const Rigid& cam, const Rigid& mod, float max_dist,
+                                   float min_dot, float* __restrict__ partial, float (*lds)[kSums + 1]) {
+  float s[kSums];
+#pragma unroll
+  for (int i = 0; i < kSums; ++i) s[i] = 0.0F;
+  unsigned int cnt = 0;
+  const int n = m.lw * m.lh;
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    float J[6], res, dist;
+    if (!match_pixel(m, cam, mod, max_dist, min_dot, p % m.lw, p / m.lw, J, res, dist)) continue;
+    int e = 0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = r; c < 6; ++c, ++e) s[e] += J[r] * J[c];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) s[21 + r] += J[r] * res;
+    s[27] += dist;
+    s[28] += res * res;
+    ++cnt;
+  }
+  // wave butterfly
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) s[i] += __shfl_xor(s[i], off);
+    cnt += __shfl_xor(cnt, off);
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < kSums; ++i) lds[wave][i] = s[i];
+    lds[wave][kSums] = __uint_as_float(cnt);
+  }
+  __syncthreads();
+  if (threadIdx.x < kSums) {
+    const int i = threadIdx.x;
+    partial[blockIdx.x * kPartialStride + i] = (lds[0][i] + lds[1][i]) + (lds[2][i] + lds[3][i]);
+  } else if (threadIdx.x == kSums) {
+    const unsigned int c = __float_as_uint(lds[0][kSums]) + __float_as_uint(lds[1][kSums]) +
+                           __float_as_uint(lds[2][kSums]) + __float_as_uint(lds[3][kSums]);
+    partial[blockIdx.x * kPartialStride + kSums] = __uint_as_float(c);
+  }
+}
+
+// Ticket protocol: returns true in the workgroup that finished last, after
+// which the partial rows of every workgroup are visible.
+__device__ inline bool last_block(unsigned int* ticket) {
+  __shared__ bool is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned int t = atomicAdd(ticket, 1U);
+    is_last = (t == gridDim.x - 1U);
+  }
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+// Kahan-compensated FP32 sum of the partial rows in block order; thread t
+// owns column t.  Result lands in out[0..29] (out[29]: count bits).
+__device__ inline void combine_partials(const float* __restrict__ partial, int nblocks, float* out) {
+  const int t = threadIdx.x;
+  if (t < kSums) {
+    float sum = 0.0F, comp = 0.0F;
+    for (int b = 0; b < nblocks; ++b) {
+      const float v = __builtin_nontemporal_load(&partial[b * kPartialStride + t]);
+      const float y = v - comp;
+      const float s2 = sum + y;
+      comp = (s2 - sum) - y;
+      sum = s2;
+    }
+    out[t] = sum;
+  } else if (t == kSums) {
+    unsigned int c = 0;
+    for (int b = 0; b < nblocks; ++b)
+      c += __float_as_uint(__builtin_nontemporal_load(&partial[b * kPartialStride + kSums]));
+    out[kSums] = __uint_as_float(c);
+  }
+}
+
+// FP64 LDL^T with symmetric diagonal pivoting (largest remaining |d| first),
+// the factorisation Eigen::LDLT uses in the reference host solve
+// (icp_optimizer.cpp:391): a pivot with |d| <= DBL_MIN contributes a zero
+// solution component, so a damped semidefinite system still solves; the
+// solve fails only on a non-finite solution.
+__device__ bool solve6(const double* eq, int damping_mode, double scale, double* x) {
+  double a[6][6];
+  int e = 0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = r; c < 6; ++c, ++e) {
+      a[r][c] = eq[e];
+      a[c][r] = eq[e];
+    }
+  for (int r = 0; r < 6; ++r) {
+    if (damping_mode == 1) a[r][r] += scale;
+    else if (damping_mode == 2) a[r][r] += scale * a[r][r];
+  }
+  int perm[6] = {0, 1, 2, 3, 4, 5};
+  double d[6];
+  for (int k = 0; k < 6; ++k) {
+    int piv = k;
+    for (int i = k + 1; i < 6; ++i)
+      if (fabs(a[i][i]) > fabs(a[piv][piv])) piv = i;
+    if (piv != k) {
+      for (int c = 0; c < 6; ++c) { const double t = a[k][c]; a[k][c] = a[piv][c]; a[piv][c] = t; }
+      for (int r = 0; r < 6; ++r) { const double t = a[r][k]; a[r][k] = a[r][piv]; a[r][piv] = t; }
+      const int tp = perm[k]; perm[k] = perm[piv]; perm[piv] = tp;
+    }
+    d[k] = a[k][k];
+    const bool zero = fabs(d[k]) <= 2.2250738585072014e-308;
+    for (int i = k + 1; i < 6; ++i) a[i][k] = zero ? 0.0 : a[i][k] / d[k];
+    for (int i = k + 1; i < 6; ++i)
+      for (int j = k + 1; j <= i; ++j) {
+        a[i][j] -= a[i][k] * d[k] * a[j][k];
+        a[j][i] = a[i][j];
+      }
+  }
+  double y[6];
+  for (int i = 0; i < 6; ++i) {
+    double s = eq[21 + perm[i]];
+    for (int q = 0; q < i; ++q) s -= a[i][q] * y[q];
+    y[i] = s;
+  }
+  for (int i = 0; i < 6; ++i) y[i] = fabs(d[i]) <= 2.2250738585072014e-308 ? 0.0 : y[i] / d[i];
+  double z[6];
+  for (int i = 5; i >= 0; --i) {
+    double s = y[i];
+    for (int q = i + 1; q < 6; ++q) s -= a[q][i] * z[q];
+    z[i] = s;
+  }
+  for (int i = 0; i < 6; ++i) x[perm[i]] = z[i];
+  for (int i = 0; i < 6; ++i)
+    if (!isfinite(x[i])) return false;
+  return true;
+}
+
+// pose <- exp([w]) pose in FP64 (Rodrigues; left-multiplied world-frame
+// increment, icp_optimizer.cpp:286).
+__device__ void compose_increment(const double* x, double* pose) {
+  const double wx = x[0], wy = x[1], wz = x[2];
+  const double angle = sqrt(wx * wx + wy * wy + wz * wz);
+  double R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (angle > 1.0e-12) {
+    const double ux = wx / angle, uy = wy / angle, uz = wz / angle;
+    double s, c;
+    sincos(angle, &s, &c);
+    const double oc = 1.0 - c;
+    R[0] = c + ux * ux * oc;
+    R[1] = ux * uy * oc - uz * s;
+    R[2] = ux * uz * oc + uy * s;
+    R[3] = uy * ux * oc + uz * s;
+    R[4] = c + uy * uy * oc;
+    R[5] = uy * uz * oc - ux * s;
+    R[6] = uz * ux * oc - uy * s;
+    R[7] = uz * uy * oc + ux * s;
+    R[8] = c + uz * uz * oc;
+  }
+  double out[12];
+  for (int r = 0; r < 3; ++r) {
+    for (int c = 0; c < 3; ++c)
+      out[r * 4 + c] = R[r * 3 + 0] * pose[0 * 4 + c] + R[r * 3 + 1] * pose[1 * 4 + c] +
+                       R[r * 3 + 2] * pose[2 * 4 + c];
+    out[r * 4 + 3] = R[r * 3 + 0] * pose[3] + R[r * 3 + 1] * pose[7] + R[r * 3 + 2] * pose[11] + x[3 + r];
+  }
+  for (int i = 0; i < 12; ++i) pose[i] = out[i];
+}
+
+__device__ void sym_eig6(const double* eq, double* ev);
+
+// check_system_stability (icp_optimizer.cpp:319-347) on FP64 sums.
+__device__ bool stable_system(const double* eq, long long count, const DeviceIcpParams& p,
+                              double* min_eig, double* cond) {
+  if (count < p.min_corr) return false;
+  double ev[6];
+  sym_eig6(eq, ev);
+  double mn = ev[0], mx = ev[0];
+  for (int i = 0; i < 6; ++i) {
+    if (!isfinite(ev[i])) return false;
+    mn = fmin(mn, ev[i]);
+    mx = fmax(mx, ev[i]);
+  }
+  if (min_eig != nullptr) *min_eig = mn;
+  if (!(mn > 0.0 && mx > 0.0)) return false;
+  if (cond != nullptr) *cond = mx / mn;
+  return mn >= static_cast<double>(p.min_eig) && mx / mn <= static_cast<double>(p.max_cond);
+}
+
+// One trial on the combined sums (solve_update_kernel semantics).  A failing
+// trial reports kUnconstrainedSystem when the undamped system is unstable,
+// as the reference host loop vetoes before solving (icp_optimizer.cpp:263-270).
+__device__ void run_trial(IcpState* st, const DeviceIcpParams& p, const float* sums32) {
+  double sums[kSums];
+  for (int i = 0; i < kSums; ++i) sums[i] = static_cast<double>(sums32[i]);
+  const long long count = static_cast<long long>(__float_as_uint(sums32[kSums]));
+  for (int i = 0; i < 30; ++i) st->sums32[i] = sums32[i];
+  if (count < p.min_corr) {
+    for (int i = 0; i < kSums; ++i) st->eq[i] = sums[i];
+    st->eq_count = count;
+    st->status = 2;
+    return;
+  }
+  if (st->trials == 0) st->lambda = p.lambda0;
+  const bool damping = p.damping_mode != 0;
+  const bool adaptive = damping && p.adaptive != 0;
+  bool accepted = true;
+  if (adaptive) {
+    const double cost = sums[28] / static_cast<double>(count);
+    accepted = st->trials == 0 || cost <= st->best_cost;
+    if (accepted) {
+      st->best_cost = cost;
+      for (int i = 0; i < kSums; ++i) st->best_eq[i] = sums[i];
+      st->best_count = count;
+      for (int i = 0; i < 16; ++i) st->best_pose[i] = st->pose[i];
+      st->lambda = fmax(st->lambda * p.dec, static_cast<double>(p.lmin));
+    } else {
+      ++st->rejected;
+      for (int i = 0; i < 16; ++i) st->pose[i] = st->best_pose[i];
+      st->lambda = fmin(st->lambda * p.inc, static_cast<double>(p.lmax));
+    }
+    for (int i = 0; i < kSums; ++i) st->eq[i] = st->best_eq[i];
+    st->eq_count = st->best_count;
+  } else {
+    for (int i = 0; i < kSums; ++i) st->eq[i] = sums[i];
+    st->eq_count = count;
+  }
+  ++st->trials;
+  const int attempts = damping ? 8 : 1;
+  double x[6] = {0, 0, 0, 0, 0, 0};
+  double scale = st->lambda, used = st->lambda, ut = 0.0, ang = 0.0;
+  bool solved = false;
+  for (int a = 0; a < attempts; ++a) {
+    double xs[6];
+    if (solve6(st->eq, p.damping_mode, scale, xs)) {
+      solved = true;
+      used = scale;
+      for (int i = 0; i < 6; ++i) x[i] = xs[i];
+      ang = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+      ut = sqrt(x[3] * x[3] + x[4] * x[4] + x[5] * x[5]);
+      if (!damping || (ut <= p.max_t && ang <= p.max_r)) break;
+    }
+    scale = fmin(scale * p.inc, static_cast<double>(p.lmax));
+  }
+  if (!solved) {
+    st->status = (!damping && !stable_system(st->eq, st->eq_count, p, nullptr, nullptr)) ? 5 : 3;
+    return;
+  }
+  st->lambda = used;
+  st->update_t = ut;
+  st->update_r = ang;
+  if (ut > p.max_t || ang > p.max_r) {
+    st->status = (!damping && !stable_system(st->eq, st->eq_count, p, nullptr, nullptr)) ? 5 : 4;
+    return;
+  }
+  compose_increment(x, st->pose);
+  if (accepted && ut < p.min_t && ang < p.min_r) st->status = 1;
+}
+
+__global__ void __launch_bounds__(256) icp_trial_kernel(Maps m, DeviceIcpParams p, IcpState* st,
+                                                         float* __restrict__ partial,
+                                                         unsigned int* __restrict__ ticket) {
+  __shared__ float lds[4][kSums + 1];
+  __shared__ float combined[kSums + 1];
+  __shared__ float xf[24];
+  if (st->status != 0 || st->frame_failed) return;  // latched: nothing to do
+  if (threadIdx.x == 0) {
+    double inv[16];
+    rigid_inverse64(st->model_pose, inv);
+    for (int i = 0; i < 12; ++i) {
+      xf[i] = static_cast<float>(st->pose[i]);
+      xf[12 + i] = static_cast<float>(inv[i]);
+    }
+  }
+  __syncthreads();
+  const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
+  sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
+  if (!last_block(ticket)) return;
+  combine_partials(partial, gridDim.x, combined);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    run_trial(st, p, combined);
+    *ticket = 0U;
+  }
+}
+
+// Single sweep with explicit float transforms (kf_icp_reduce).
+__global__ void __launch_bounds__(256) icp_sweep_kernel(Maps m, Rigid cam, Rigid mod, float max_dist,
+                                                         float min_dot, IcpState* st,
+                                                         float* __restrict__ partial,
+                                                         unsigned int* __restrict__ ticket) {
+  __shared__ float lds[4][kSums + 1];
+  __shared__ float combined[kSums + 1];
+  sweep_block(m, cam, mod, max_dist, min_dot, partial, lds);
+  if (!last_block(ticket)) return;
+  combine_partials(partial, gridDim.x, combined);
+  __syncthreads();
+  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
+  if (threadIdx.x == 0) *ticket = 0U;
+}
+
+// FP64 cyclic Jacobi eigenvalues of the symmetric 6x6 system.
+__device__ void sym_eig6(const double* eq, double* ev) {
+  double a[6][6];
+  int e = 0;
+  for (int r = 0; r < 6; ++r)
+    for (int c = r; c < 6; ++c, ++e) {
+      a[r][c] = eq[e];
+      a[c][r] = eq[e];
+    }
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    double off = 0.0;
+    for (int pp = 0; pp < 6; ++pp)
+      for (int q = pp + 1; q < 6; ++q) off += a[pp][q] * a[pp][q];
+    if (off == 0.0) break;
+    for (int pp = 0; pp < 6; ++pp)
+      for (int q = pp + 1; q < 6; ++q) {
+        if (a[pp][q] == 0.0) continue;
+        const double theta = (a[q][q] - a[pp][pp]) / (2.0 * a[pp][q]);
+        const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        const double c = 1.0 / sqrt(t * t + 1.0);
+        const double s = t * c;
+        for (int k2 = 0; k2 < 6; ++k2) {
+          const double akp = a[k2][pp], akq = a[k2][q];
+          a[k2][pp] = c * akp - s * akq;
+          a[k2][q] = s * akp + c * akq;
+        }
+        for (int k2 = 0; k2 < 6; ++k2) {
+          const double apk = a[pp][k2], aqk = a[q][k2];
+          a[pp][k2] = c * apk - s * aqk;
+          a[q][k2] = s * apk + c * aqk;
+        }
+      }
+  }
+  for (int i = 0; i < 6; ++i) ev[i] = a[i][i];
+}
+
+// Prepare a level: status cleared, model pose = running pose.
+__global__ void icp_begin_level_kernel(IcpState* st, int level) {
+  if (threadIdx.x != 0) return;
+  kf_icp_result& r = st->results[level];
+  r = kf_icp_result{};
+  r.status = -1;
+  if (st->frame_failed) return;
+  st->status = 0;
+  st->trials = 0;
+  st->rejected = 0;
+  st->best_cost = 0.0;
+  st->lambda = 0.0;
+  st->eq_count = 0;
+  st->best_count = 0;
+  st->update_t = 0.0;
+  st->update_r = 0.0;
+  for (int i = 0; i < 16; ++i) {
+    st->model_pose[i] = st->pose[i];
+    st->level_pose0[i] = st->pose[i];
+    st->best_pose[i] = st->pose[i];
+  }
+}
+
+// Stability check and result record (icp_optimizer.cpp:108-169).
+__global__ void icp_finish_level_kernel(IcpState* st, DeviceIcpParams p, int level) {
+  if (threadIdx.x != 0) return;
+  if (st->frame_failed) return;
+  kf_icp_result& r = st->results[level];
+  int status = st->status;
+  r.min_eigenvalue = 0.0;
+  r.condition_number = 0.0;
+  if (status == 0 || status == 1) {
+    const bool stable = stable_system(st->eq, st->eq_count, p, &r.min_eigenvalue, &r.condition_number);
+    if (!stable && p.damping_mode == 0) status = 5;
+  }
+  r.status = status;
+  r.trials = st->trials;
+  r.rejected = st->rejected;
+  r.correspondences = st->eq_count;
+  if (st->eq_count > 0) {
+    r.mean_distance = st->eq[27] / static_cast<double>(st->eq_count);
+    r.mean_squared_residual = st->eq[28] / static_cast<double>(st->eq_count);
+  }
+  r.update_translation = st->update_t;
+  r.update_rotation = st->update_r;
+  r.lambda = st->lambda;
+  st->status = status;
+  if (status >= 2) {
+    // failure: the outcome keeps the level's initial pose
+    for (int i = 0; i < 16; ++i) st->pose[i] = st->level_pose0[i];
+    st->frame_failed = 1;
+    st->failed_level = level;
+  }
+}
+
+Maps make_maps(const float4* lv_v, const float4* lv_n, int lw, int lh, const float4* md_v,
+               const float4* md_n, int mw, int mh, Intr k) {
+  return Maps{lv_v, lv_n, md_v, md_n, lw, lh, mw, mh, k};
+}
+
+int blocks_for(int n) { return std::max(1, std::min(kMaxBlocks, (n + 255) / 256)); }
+
+}  // namespace
+
+DeviceIcpParams to_device_params(const kf_icp_options& o) {
+  DeviceIcpParams p{};
+  p.min_corr = o.min_correspondences;
+  p.max_dist = o.max_point_distance;
+  p.min_dot = o.min_normal_dot;
+  p.min_t = o.min_update_translation;
+  p.min_r = o.min_update_rotation;
+  p.max_t = o.max_update_translation;
+  p.max_r = o.max_update_rotation;
+  p.min_eig = o.min_system_eigenvalue;
+  p.max_cond = o.max_condition_number;
+  p.damping_mode = o.damping_mode;
+  p.lambda0 = o.damping_lambda;
+  p.adaptive = o.adaptive_damping;
+  p.inc = o.lambda_increase;
+  p.dec = o.lambda_decrease;
+  p.lmin = o.min_lambda;
+  p.lmax = o.max_lambda;
+  return p;
+}
+
+void launch_icp_begin_level(kf_context* ctx, int level) {
+  hipLaunchKernelGGL(icp_begin_level_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, level);
+  KF_HIP(hipGetLastError());
+}
+
+void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
+                      const float4* md_v, const float4* md_n, int mw, int mh, Intr k, int iterations,
+                      const DeviceIcpParams& p, int level) {
+  const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
+  const int nb = blocks_for(lw * lh);
+  for (int it = 0; it < iterations; ++it) {
+    hipLaunchKernelGGL(icp_trial_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
+                       ctx->partials, ctx->ticket);
+  }
+  hipLaunchKernelGGL(icp_finish_level_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, p, level);
+  KF_HIP(hipGetLastError());
+}
+
+void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
+                            const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
+                            const float* camera12, const float* model12, float max_dist,
+                            float min_dot) {
+  const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
+  const int nb = blocks_for(lw * lh);
+  hipLaunchKernelGGL(icp_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m,
+                     rigid_from12(camera12), rigid_from12(model12), max_dist, min_dot, ctx->icp,
+                     ctx->partials, ctx->ticket);
+  KF_HIP(hipGetLastError());
+}
+
+}  // namespace kf

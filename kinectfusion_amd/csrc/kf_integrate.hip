@@ -1,0 +1,354 @@
+// Dense TSDF integration (reference: IntegrationContext::observe/fuse and the
+// Classic / AngleWeighted rules, include/kinectfusion/tsdf_integration.hpp:
+// 183-342; full sweep :356-370; BasicVoxel::fused, volume.hpp:137-146).
+//
+// Three kernels per frame:
+//  1. pixel_prep  -- everything observe() derives from the pixel alone:
+//     measured depth, |K^-1 [u v 1]| (lambda), the rule's observation weight
+//     (angle rule: cos(theta)/depth from the level-0 normal), the truncation
+//     band; plus the maximum usable depth of every 16 x 16 pixel tile.
+//  2. brick_cull  -- one thread per 64 x 4 x 4 voxel brick: a brick is kept
+//     unless no voxel of it can project into the image in front of the
+//     camera, or its nearest voxel lies farther than the deepest depth of
+//     the tiles it covers plus the truncation band.  The bounds carry
+//     margins so that no voxel the full sweep would fuse is ever culled: the
+//     result is bit-identical to the full sweep.  Kept bricks are compacted
+//     into a list with one atomic per wave.
+//  3. integrate   -- persistent workgroups walk the list; a wave covers one
+//     64-voxel x-row of a brick, so voxel loads and stores are contiguous
+//     (512 B per wave for 8 B voxels, 256 B for 4 B voxels).
+#include "kf_internal.hpp"
+
+namespace kf {
+namespace {
+
+struct FrameParams {
+  Rigid w2c;
+  Intr k;
+  int w, h;
+  float scale, lo, hi;
+  float obs_weight, max_weight;
+  int projective;
+  int dist_scaled;
+  float trunc_scale;
+  int angle_rule;
+  const IcpState* st;  // when set: w2c from the running device pose; skip on failure
+};
+
+// world_to_camera of this frame, computed once per workgroup.
+__device__ inline bool block_w2c(const FrameParams& f, Rigid& out) {
+  __shared__ float xf[12];
+  __shared__ int skip;
+  if (f.st == nullptr) {
+    out = f.w2c;
+    return true;
+  }
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    skip = f.st->frame_failed;
+    double inv[16];
+    rigid_inverse64(f.st->pose, inv);
+    for (int i = 0; i < 12; ++i) xf[i] = static_cast<float>(inv[i]);
+  }
+  __syncthreads();
+  out = rigid_from12(xf);
+  return skip == 0;
+}
+
+// 1. per-pixel record {depth m (or -1), lambda, weight, truncation}.
+__global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restrict__ depth,
+                                                          const float4* __restrict__ normals,
+                                                          float4* __restrict__ pix,
+                                                          float* __restrict__ tile_max,
+                                                          FrameParams f, float base_trunc) {
+  __shared__ float red[256];
+  const int col = blockIdx.x * kTile + threadIdx.x, row = blockIdx.y * kTile + threadIdx.y;
+  const int tid = threadIdx.y * kTile + threadIdx.x;
+  float dmax = 0.0F;
+  if (col < f.w && row < f.h) {
+    const size_t i = static_cast<size_t>(row) * f.w + col;
+    float d;
+    float4 rec = make_float4(-1.0F, 0.0F, 0.0F, 0.0F);
+    if (depth_in_range(depth[i], f.scale, f.lo, f.hi, d)) {
+      const V3 ray = back_project(f.k, static_cast<float>(col), static_cast<float>(row), 1.0F);
+      const float lambda = norm3(ray);
+      if (finite3(ray) && lambda != 0.0F) {
+        float w = f.obs_weight;
+        if (f.angle_rule && normals != nullptr) {
+          const V3 n = ld3(normals[i]);
+          if (finite3(n)) {
+            const V3 nr = normalize3(ray);
+            const V3 view = mk3(-nr.x, -nr.y, -nr.z);
+            const float cos_theta = fmax_ref(0.0F, dot3(normalize3(n), view));
+            w *= cos_theta / d;
+          }
+        }
+        const float trunc = f.dist_scaled ? base_trunc + (f.trunc_scale * d) : base_trunc;
+        rec = make_float4(d, lambda, w, trunc);
+        dmax = d;
+      }
+    }
+    pix[i] = rec;
+  }
+  red[tid] = dmax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  if (tid == 0) tile_max[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// 2. conservative brick cull.
+__global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
+                                                          const float* __restrict__ tile_max,
+                                                          int tiles_x, int tiles_y,
+                                                          uint32_t* __restrict__ list,
+                                                          uint32_t* __restrict__ count) {
+  const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
+  const int bnz = (g.nz + kBrickZ - 1) / kBrickZ;
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  Rigid w2c;
+  if (!block_w2c(f, w2c)) return;
+  bool keep = false;
+  if (b < static_cast<uint32_t>(bnx * bny * bnz)) {
+    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
+    const int xs[2] = {bx * kBrickX, min(bx * kBrickX + kBrickX - 1, g.nx - 1)};
+    const int ys[2] = {by * kBrickY, min(by * kBrickY + kBrickY - 1, g.ny - 1)};
+    const int zs[2] = {bz * kBrickZ, min(bz * kBrickZ + kBrickZ - 1, g.nz - 1)};
+    float zmin = 3.0e38F, zmax = -3.0e38F, umin = 3.0e38F, umax = -3.0e38F, vmin = 3.0e38F,
+          vmax = -3.0e38F;
+    V3 cam[8];
+    for (int c = 0; c < 8; ++c) {
+      cam[c] = xform(w2c, g.center(xs[c & 1], ys[(c >> 1) & 1], zs[c >> 2]));
+      zmin = fminf(zmin, cam[c].z);
+      zmax = fmaxf(zmax, cam[c].z);
+    }
+    keep = zmax > -1.0e-4F;
+    int tx0 = 0, tx1 = tiles_x - 1, ty0 = 0, ty1 = tiles_y - 1;
+    if (keep && zmin > 1.0e-3F) {
+      for (int c = 0; c < 8; ++c) {
+        float u, v;
+        project(f.k, cam[c], u, v);
+        umin = fminf(umin, u);
+        umax = fmaxf(umax, u);
+        vmin = fminf(vmin, v);
+        vmax = fmaxf(vmax, v);
+      }
+      // a voxel lands on pixel round(u): usable columns are [0, w); keep a
+      // two-pixel margin against rounding of the hull vertices.
+      umin -= 2.0F;
+      vmin -= 2.0F;
+      umax += 2.0F;
+      vmax += 2.0F;
+      if (umax < -0.5F || vmax < -0.5F || umin > static_cast<float>(f.w) - 0.5F ||
+          vmin > static_cast<float>(f.h) - 0.5F) {
+        keep = false;
+      } else {
+        tx0 = max(0, static_cast<int>(floorf((fmaxf(umin, -1.0F) + 0.5F) / kTile)));
+        ty0 = max(0, static_cast<int>(floorf((fmaxf(vmin, -1.0F) + 0.5F) / kTile)));
+        tx1 = min(tiles_x - 1, static_cast<int>(floorf((fminf(umax, 1.0e6F) + 0.5F) / kTile)));
+        ty1 = min(tiles_y - 1, static_cast<int>(floorf((fminf(vmax, 1.0e6F) + 0.5F) / kTile)));
+      }
+    }
+    if (keep) {
+      float dmax = 0.0F;
+      for (int ty = ty0; ty <= ty1; ++ty)
+        for (int tx = tx0; tx <= tx1; ++tx) dmax = fmaxf(dmax, tile_max[ty * tiles_x + tx]);
+      if (!(dmax > 0.0F)) {
+        keep = false;
+      } else if (zmin > 1.0e-3F) {
+        // |p| / lambda >= z (1 - 0.71 / fx) for a voxel rounded to its pixel;
+        // fusion needs |p| / lambda <= d + truncation.
+        const float trunc = f.dist_scaled ? g.trunc + f.trunc_scale * dmax : g.trunc;
+        const float shrink = 1.0F - 1.0F / fmaxf(f.k.fx, 1.0F);
+        if (zmin * shrink > dmax + trunc + 1.0e-3F) keep = false;
+      }
+    }
+  }
+  // wave-aggregated append
+  const unsigned long long mask = __ballot(keep);
+  const int lane = threadIdx.x & 63;
+  uint32_t base = 0;
+  if (lane == 0 && mask != 0ULL) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, 0);
+  if (keep) {
+    const unsigned long long below = mask & ((1ULL << lane) - 1ULL);
+    list[base + __popcll(below)] = b;
+  }
+}
+
+// 3. integrate the listed bricks.
+template <int Kind, bool Color, bool Count>
+__global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
+                                                         const float4* __restrict__ pix,
+                                                         const uint32_t* __restrict__ color_img,
+                                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
+                                                         float4* __restrict__ colors,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ count,
+                                                         unsigned long long* __restrict__ fused) {
+  using IO = VoxelIO<Kind>;
+  Rigid w2c;
+  if (!block_w2c(f, w2c)) return;
+  const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
+  const uint32_t n = *count;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  unsigned int my_fused = 0, my_color = 0;
+  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+    const uint32_t b = list[li];
+    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
+    const int x = bx * kBrickX + tx, y = by * kBrickY + ty;
+    if (x >= g.nx || y >= g.ny) continue;
+    for (int zz = 0; zz < kBrickZ; ++zz) {
+      const int z = bz * kBrickZ + zz;
+      if (z >= g.nz) break;
+      const V3 cp = xform(w2c, g.center(x, y, z));
+      if (cp.z <= 0.0F) continue;
+      float u, v;
+      project(f.k, cp, u, v);
+      int col, row;
+      if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) continue;
+      const size_t pi = static_cast<size_t>(row) * f.w + col;
+      const float4 p = pix[pi];
+      if (p.x < 0.0F) continue;
+      const float sd = f.projective ? p.x - (norm3(cp) / p.y) : p.x - cp.z;
+      if (sd < -p.w) continue;
+      const float tsdf = fclamp_ref(sd / p.w, -1.0F, 1.0F);
+      const float w = p.z;
+      if (w <= 0.0F) continue;
+      const size_t vi = g.flat(x, y, z);
+      float t0, w0;
+      bool obs, fin;
+      IO::unpack(vox[vi], t0, w0, obs, fin);
+      const float avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
+      const float tw = fmin_ref(w0 + w, f.max_weight);
+      vox[vi] = IO::pack(avg, tw);
+      if (Count) ++my_fused;
+      if (Color) {
+        if (color_img != nullptr && sd <= p.w * 0.5F) {
+          const uint32_t px = color_img[pi];
+          const V3 c = mk3(static_cast<float>(px & 0xFFU), static_cast<float>((px >> 8U) & 0xFFU),
+                           static_cast<float>((px >> 16U) & 0xFFU));
+          const float4 old = colors[vi];
+          const float cw = old.w;
+          const float den = cw + w;
+          colors[vi] = make_float4(((old.x * cw) + (c.x * w)) / den, ((old.y * cw) + (c.y * w)) / den,
+                                   ((old.z * cw) + (c.z * w)) / den, fmin_ref(cw + w, f.max_weight));
+          if (Count) ++my_color;
+        }
+      }
+    }
+  }
+  if (Count) {
+    for (int off = 32; off > 0; off >>= 1) {
+      my_fused += __shfl_xor(my_fused, off);
+      my_color += __shfl_xor(my_color, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&fused[0], static_cast<unsigned long long>(my_fused));
+      atomicAdd(&fused[1], static_cast<unsigned long long>(my_color));
+    }
+  }
+}
+
+__global__ void zero_counters_kernel(uint32_t* c, unsigned long long* f) {
+  if (threadIdx.x < 4) c[threadIdx.x] = 0U;
+  if (threadIdx.x < 2) f[threadIdx.x] = 0ULL;
+}
+
+template <int Kind>
+__global__ void fill_voxels_kernel(typename VoxelIO<Kind>::Raw* v, float4* colors, size_t n) {
+  const typename VoxelIO<Kind>::Raw fresh = VoxelIO<Kind>::fresh();
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    v[i] = fresh;
+    if (colors != nullptr) colors[i] = make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+  }
+}
+
+template <int Kind, bool Color>
+void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
+  using Raw = typename VoxelIO<Kind>::Raw;
+  const int grid = 2048;
+  if (count) {
+    hipLaunchKernelGGL((integrate_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
+                       ctx->colors, ctx->bricks, ctx->counters, ctx->fused);
+  } else {
+    hipLaunchKernelGGL((integrate_kernel<Kind, Color, false>), dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
+                       ctx->colors, ctx->bricks, ctx->counters, ctx->fused);
+  }
+}
+
+}  // namespace
+
+void launch_fill_voxels(kf_context* ctx) {
+  const int grid = 4096;
+  switch (ctx->cfg.voxel_store) {
+    case KF_VOXEL_QUANTIZED:
+      hipLaunchKernelGGL(fill_voxels_kernel<KF_VOXEL_QUANTIZED>, dim3(grid), dim3(256), 0,
+                         ctx->stream, static_cast<uint32_t*>(ctx->voxels), ctx->colors, ctx->n_voxels);
+      break;
+    case KF_VOXEL_BF16:
+      hipLaunchKernelGGL(fill_voxels_kernel<KF_VOXEL_BF16>, dim3(grid), dim3(256), 0, ctx->stream,
+                         static_cast<uint32_t*>(ctx->voxels), ctx->colors, ctx->n_voxels);
+      break;
+    default:
+      hipLaunchKernelGGL(fill_voxels_kernel<KF_VOXEL_FLOAT>, dim3(grid), dim3(256), 0, ctx->stream,
+                         static_cast<float2*>(ctx->voxels), ctx->colors, ctx->n_voxels);
+      break;
+  }
+  KF_HIP(hipGetLastError());
+}
+
+void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state) {
+  const kf_config& c = ctx->cfg;
+  FrameParams f{};
+  f.w2c = rigid_from12(fr.world_to_camera);
+  f.k = Intr{fr.fx, fr.fy, fr.cx, fr.cy};
+  f.w = fr.width;
+  f.h = fr.height;
+  f.scale = c.depth_scale;
+  f.lo = c.min_depth;
+  f.hi = c.max_depth;
+  f.obs_weight = c.observation_weight;
+  f.max_weight = c.max_weight;
+  f.projective = c.projective_distance;
+  f.dist_scaled = c.distance_scaled_truncation;
+  f.trunc_scale = c.truncation_distance_scale;
+  f.angle_rule = c.tsdf_rule == KF_RULE_ANGLE_WEIGHTED;
+  f.st = from_state ? ctx->icp : nullptr;
+  const int tiles_x = (f.w + kTile - 1) / kTile, tiles_y = (f.h + kTile - 1) / kTile;
+  // per-pixel and per-tile scratch is sized for the configured camera
+  if (f.w > c.width || f.h > c.height)
+    fail(KF_ERR_INVALID_ARGUMENT, "integrate: frame larger than the configured camera");
+  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, ctx->fused);
+  hipLaunchKernelGGL(pixel_prep_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream,
+                     fr.depth, reinterpret_cast<const float4*>(fr.normals), ctx->pix, ctx->tile_max,
+                     f, ctx->grid.trunc);
+  const Grid& g = ctx->grid;
+  const int nb = ((g.nx + kBrickX - 1) / kBrickX) * ((g.ny + kBrickY - 1) / kBrickY) *
+                 ((g.nz + kBrickZ - 1) / kBrickZ);
+  hipLaunchKernelGGL(brick_cull_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, g, f,
+                     ctx->tile_max, tiles_x, tiles_y, ctx->bricks, ctx->counters);
+  const bool color = ctx->colors != nullptr;
+  const uint32_t* cimg = fr.color;
+  switch (c.voxel_store) {
+    case KF_VOXEL_QUANTIZED:
+      color ? launch_integrate_t<KF_VOXEL_QUANTIZED, true>(ctx, f, cimg, count)
+            : launch_integrate_t<KF_VOXEL_QUANTIZED, false>(ctx, f, cimg, count);
+      break;
+    case KF_VOXEL_BF16:
+      color ? launch_integrate_t<KF_VOXEL_BF16, true>(ctx, f, cimg, count)
+            : launch_integrate_t<KF_VOXEL_BF16, false>(ctx, f, cimg, count);
+      break;
+    default:
+      color ? launch_integrate_t<KF_VOXEL_FLOAT, true>(ctx, f, cimg, count)
+            : launch_integrate_t<KF_VOXEL_FLOAT, false>(ctx, f, cimg, count);
+      break;
+  }
+  KF_HIP(hipGetLastError());
+}
+
+}  // namespace kf

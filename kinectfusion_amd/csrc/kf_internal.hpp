@@ -1,0 +1,327 @@
+// Internal declarations of libkf_hip: the context, device buffers and the
+// float math shared by the kernels.
+//
+// Bit-exact parity with the reference CPU path rests on three rules kept by
+// every kernel here:
+//  * the same float operation order as the reference host code (dot is
+//    (x*x + y*y) + z*z, include/kinectfusion/vector.hpp:143-146; cell centres
+//    are origin + (i + 0.5) * voxel, volume.hpp:276-283; projection is
+//    fx * x / z + cx, rgbd.hpp:40-44);
+//  * IEEE division and square root and no FMA contraction (the build uses
+//    -ffp-contract=off and hip's correctly rounded fp32 divide/sqrt);
+//  * the host form of normalized(): divide by the length (vector.hpp:164-174).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "../../include/kf_hip.h"
+
+namespace kf {
+
+constexpr int kMaxLevels = 4;
+constexpr int kBrickX = 64;  // integration brick: 64 x 4 x 4 voxels
+constexpr int kBrickY = 4;
+constexpr int kBrickZ = 4;
+constexpr int kTile = 16;    // depth-tile edge for the integration cull
+
+// ---------------------------------------------------------------------------
+// Float vector math (host + device).
+// ---------------------------------------------------------------------------
+struct V3 {
+  float x, y, z;
+};
+__host__ __device__ inline V3 mk3(float x, float y, float z) { return V3{x, y, z}; }
+__host__ __device__ inline V3 add3(V3 a, V3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline V3 sub3(V3 a, V3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline V3 scale3(V3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline V3 div3(V3 a, float s) { return mk3(a.x / s, a.y / s, a.z / s); }
+__host__ __device__ inline float dot3(V3 a, V3 b) { return (a.x * b.x) + (a.y * b.y) + (a.z * b.z); }
+__host__ __device__ inline V3 cross3(V3 a, V3 b) {
+  return mk3((a.y * b.z) - (a.z * b.y), (a.z * b.x) - (a.x * b.z), (a.x * b.y) - (a.y * b.x));
+}
+__host__ __device__ inline float norm3(V3 v) { return sqrtf(dot3(v, v)); }
+__host__ __device__ inline V3 normalize3(V3 v) {
+  const float len = norm3(v);
+  return len > 0.0F ? div3(v, len) : mk3(0.0F, 0.0F, 0.0F);
+}
+__host__ __device__ inline bool finite3(V3 v) {
+  return isfinite(v.x) && isfinite(v.y) && isfinite(v.z);
+}
+__host__ __device__ inline float fmin_ref(float a, float b) { return b < a ? b : a; }
+__host__ __device__ inline float fmax_ref(float a, float b) { return a < b ? b : a; }
+__host__ __device__ inline float fclamp_ref(float v, float lo, float hi) {
+  if (v < lo) return lo;
+  if (hi < v) return hi;
+  return v;
+}
+__host__ __device__ inline V3 ld3(const float4& v) { return mk3(v.x, v.y, v.z); }
+__host__ __device__ inline float4 st3(V3 v) { return make_float4(v.x, v.y, v.z, 0.0F); }
+__device__ inline V3 nan3() {
+  const float n = __builtin_nanf("");
+  return mk3(n, n, n);
+}
+
+// Rigid transform [R | t], rows of R.
+struct Rigid {
+  V3 r0, r1, r2, t;
+};
+__host__ __device__ inline V3 rot(const Rigid& m, V3 p) {
+  return mk3(dot3(m.r0, p), dot3(m.r1, p), dot3(m.r2, p));
+}
+__host__ __device__ inline V3 xform(const Rigid& m, V3 p) { return add3(rot(m, p), m.t); }
+__host__ __device__ inline Rigid rigid_from12(const float* m) {
+  return Rigid{mk3(m[0], m[1], m[2]), mk3(m[4], m[5], m[6]), mk3(m[8], m[9], m[10]),
+               mk3(m[3], m[7], m[11])};
+}
+__host__ __device__ inline Rigid rigid_from_pose64(const double* p) {
+  auto f = [&](int i) { return static_cast<float>(p[i]); };
+  return Rigid{mk3(f(0), f(1), f(2)), mk3(f(4), f(5), f(6)), mk3(f(8), f(9), f(10)),
+               mk3(f(3), f(7), f(11))};
+}
+// out = [R^T, -R^T t] in FP64.
+__host__ __device__ inline void rigid_inverse64(const double* a, double* out) {
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) out[r * 4 + c] = a[c * 4 + r];
+  for (int r = 0; r < 3; ++r)
+    out[r * 4 + 3] = -(out[r * 4 + 0] * a[3] + out[r * 4 + 1] * a[7] + out[r * 4 + 2] * a[11]);
+  out[12] = 0.0;
+  out[13] = 0.0;
+  out[14] = 0.0;
+  out[15] = 1.0;
+}
+
+struct Intr {
+  float fx, fy, cx, cy;
+};
+__host__ __device__ inline void project(const Intr& k, V3 p, float& u, float& v) {
+  u = (k.fx * p.x / p.z) + k.cx;
+  v = (k.fy * p.y / p.z) + k.cy;
+}
+__host__ __device__ inline V3 back_project(const Intr& k, float px, float py, float d) {
+  return mk3((px - k.cx) * d / k.fx, (py - k.cy) * d / k.fy, d);
+}
+// lround(v) accepted only when it lands in [0, limit).  roundf rounds half
+// away from zero like std::lround; NaN fails the comparison.
+__host__ __device__ inline bool round_index(float v, int limit, int& out) {
+  const float r = roundf(v);
+  if (!(r >= 0.0F && r < static_cast<float>(limit))) return false;
+  out = static_cast<int>(r);
+  return true;
+}
+__host__ __device__ inline bool depth_in_range(uint16_t raw, float scale, float lo, float hi,
+                                               float& m) {
+  if (raw == 0) return false;
+  m = static_cast<float>(raw) / scale;
+  return !(m < lo || m > hi);
+}
+
+// ---------------------------------------------------------------------------
+// Voxel stores (include/kinectfusion/volume.hpp:28-153).
+// ---------------------------------------------------------------------------
+constexpr float kQScale = 32767.0F;
+constexpr float kQInvScale = 1.0F / 32767.0F;
+constexpr float kWScale = 256.0F;
+constexpr float kWInvScale = 1.0F / 256.0F;
+constexpr float kWMax = 65535.0F / 256.0F;
+
+__host__ __device__ inline uint32_t q_encode(float t) {
+  return static_cast<uint16_t>(static_cast<int16_t>(fclamp_ref(t, -1.0F, 1.0F) * kQScale));
+}
+__host__ __device__ inline float q_decode(uint32_t raw16) {
+  return static_cast<float>(static_cast<int16_t>(static_cast<uint16_t>(raw16))) * kQInvScale;
+}
+__host__ __device__ inline uint32_t bf_encode(float t) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, fclamp_ref(t, -1.0F, 1.0F));
+  return ((b + 0x7FFFU + ((b >> 16U) & 1U)) >> 16U) & 0xFFFFU;
+}
+__host__ __device__ inline float bf_decode(uint32_t raw16) {
+  return __builtin_bit_cast(float, (raw16 & 0xFFFFU) << 16U);
+}
+__host__ __device__ inline uint32_t w_encode(float w) {
+  return static_cast<uint16_t>(fclamp_ref(w, 0.0F, kWMax) * kWScale);
+}
+__host__ __device__ inline float w_decode(uint32_t raw16) {
+  return static_cast<float>(raw16 & 0xFFFFU) * kWInvScale;
+}
+
+// Float view of a stored voxel: tsdf, weight, observed, finite.
+template <int Kind>
+struct VoxelIO;
+
+template <>
+struct VoxelIO<KF_VOXEL_FLOAT> {
+  using Raw = float2;
+  __device__ static inline void unpack(Raw v, float& t, float& w, bool& obs, bool& fin) {
+    t = v.x;
+    w = v.y;
+    obs = v.y > 0.0F;
+    fin = isfinite(v.x);
+  }
+  __device__ static inline Raw pack(float t, float w) { return make_float2(t, w); }
+  __host__ __device__ static inline Raw fresh() { return make_float2(1.0F, 0.0F); }
+};
+
+template <>
+struct VoxelIO<KF_VOXEL_QUANTIZED> {
+  using Raw = uint32_t;  // low 16 bits int16 tsdf, high 16 bits uint16 weight
+  __device__ static inline void unpack(Raw v, float& t, float& w, bool& obs, bool& fin) {
+    t = q_decode(v & 0xFFFFU);
+    w = w_decode(v >> 16U);
+    obs = (v >> 16U) != 0U;
+    fin = true;
+  }
+  __device__ static inline Raw pack(float t, float w) { return q_encode(t) | (w_encode(w) << 16U); }
+  __host__ __device__ static inline Raw fresh() { return q_encode(1.0F); }
+};
+
+template <>
+struct VoxelIO<KF_VOXEL_BF16> {
+  using Raw = uint32_t;  // low 16 bits bf16 tsdf, high 16 bits uint16 weight
+  __device__ static inline void unpack(Raw v, float& t, float& w, bool& obs, bool& fin) {
+    t = bf_decode(v & 0xFFFFU);
+    w = w_decode(v >> 16U);
+    obs = (v >> 16U) != 0U;
+    fin = true;
+  }
+  __device__ static inline Raw pack(float t, float w) { return bf_encode(t) | (w_encode(w) << 16U); }
+  __host__ __device__ static inline Raw fresh() { return bf_encode(1.0F); }
+};
+
+inline size_t voxel_bytes(int kind) { return kind == KF_VOXEL_FLOAT ? 8 : 4; }
+
+// Dense volume geometry passed to kernels by value.
+struct Grid {
+  int nx, ny, nz;
+  float voxel;
+  float ox, oy, oz;
+  float trunc;
+  __host__ __device__ inline size_t flat(int x, int y, int z) const {
+    return (static_cast<size_t>(z) * ny + y) * nx + x;
+  }
+  __host__ __device__ inline V3 center(int x, int y, int z) const {
+    return add3(mk3(ox, oy, oz),
+                scale3(add3(mk3(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z)),
+                            mk3(0.5F, 0.5F, 0.5F)),
+                       voxel));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// ICP device state, one per context (device memory).
+// ---------------------------------------------------------------------------
+struct IcpState {
+  double pose[16];        // running camera_to_world
+  double model_pose[16];  // render pose of the current level's model maps
+  double best_pose[16];
+  double eq[29];
+  double best_eq[29];
+  double best_cost;
+  double lambda;
+  double update_t, update_r;
+  double min_eig, cond;
+  long long eq_count, best_count;
+  int status;     // per level; see kf_icp_result
+  int trials, rejected;
+  int frame_failed;
+  int failed_level;
+  int pad;
+  float sums32[30];  // last compensated FP32 reduction (29 sums + count)
+  double level_pose0[16];  // pose at the start of the current level
+  kf_icp_result results[kMaxLevels];
+  double prev_pose[16];   // camera_to_world of the previous two frames
+  double prev2_pose[16];
+  long long frames_done;
+};
+
+struct DeviceIcpParams {
+  long long min_corr;
+  float max_dist, min_dot;
+  float min_t, min_r, max_t, max_r;
+  float min_eig, max_cond;
+  int damping_mode;
+  float lambda0;
+  int adaptive;
+  float inc, dec, lmin, lmax;
+};
+
+struct LevelBuffers {
+  int w = 0, h = 0;
+  Intr k{};
+  uint16_t* depth = nullptr;  // level depth (level 0: filtered)
+  float4* vert = nullptr;
+  float4* norm = nullptr;
+  float4* model_pts = nullptr;  // raycast output for this level
+  float4* model_nrm = nullptr;
+  uint32_t* model_col = nullptr;
+};
+
+}  // namespace kf
+
+struct kf_context {
+  kf_config cfg{};
+  hipStream_t stream = nullptr;
+  int device = 0;
+  std::string error;
+  kf::Grid grid{};
+  size_t n_voxels = 0;
+  void* voxels = nullptr;
+  float4* colors = nullptr;
+  int levels_built = 0;
+  kf::LevelBuffers lv[kf::kMaxLevels];
+  uint16_t* raw_depth = nullptr;  // raw depth copy for integration
+  // integration scratch
+  float4* pix = nullptr;          // per pixel: depth m, lambda, weight, valid
+  float* tile_max = nullptr;      // per 16x16 tile max usable depth
+  uint32_t* bricks = nullptr;     // active brick list
+  uint32_t* counters = nullptr;   // [0] brick count, [1..2] fused counts (u64 pair as 2x u32), spare
+  unsigned long long* fused = nullptr;
+  // icp scratch
+  float* partials = nullptr;      // per-block partial sums (32 floats each)
+  unsigned int* ticket = nullptr;
+  kf::IcpState* icp = nullptr;    // device
+  kf::IcpState* icp_host = nullptr;  // pinned mirror for results
+  double* trajectory = nullptr;   // device, 16 doubles per frame
+  int64_t traj_capacity = 0;
+  int64_t frames = 0;
+  // timing
+  hipEvent_t ev[12] = {};  // 0 start, 1 pyramid, 2.. per level (raycast, icp), 11 end
+  kf_stage_times last_times{};
+  bool have_times = false;
+  // learned warm start (pose network), created on demand
+  void* posenet = nullptr;
+};
+
+namespace kf {
+
+// Error plumbing: every ABI call runs inside guard(); HIP failures throw a
+// local exception type that the guard turns into a status and a message.
+struct Failure {
+  kf_status status;
+  std::string message;
+};
+
+[[noreturn]] void fail(kf_status status, const std::string& message);
+void hip_check(hipError_t err, const char* what);
+#define KF_HIP(expr) ::kf::hip_check((expr), #expr)
+
+// Launch helpers implemented per stage.
+void launch_bilateral(kf_context* ctx, const uint16_t* raw);
+void launch_level_maps(kf_context* ctx, int level);
+void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state);
+void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
+                    bool skip_if_failed);
+void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
+                      const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
+                      int iterations, const DeviceIcpParams& p, int level_index);
+void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
+                            const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
+                            const float* camera12, const float* model12, float max_dist,
+                            float min_dot);
+void launch_icp_begin_level(kf_context* ctx, int level_index);
+void launch_fill_voxels(kf_context* ctx);
+DeviceIcpParams to_device_params(const kf_icp_options& o);
+
+}  // namespace kf
