@@ -8,7 +8,7 @@
 // Corner blends visit the 2^3 stencil in z -> y -> x order with the
 // reference's weight products, so every sample is bit-identical to the CPU
 // sampler.
-#include "kf_internal.hpp"
+#include "kf_sampler.hpp"
 
 namespace kf {
 namespace {
@@ -21,149 +21,13 @@ struct RayParams {
   int cell_normals;
 };
 
-template <int Kind>
-struct Volume {
-  using IO = VoxelIO<Kind>;
-  const typename IO::Raw* __restrict__ vox;
-  const float4* __restrict__ colors;
-  Grid g;
-  float inv_voxel;
-
-  __device__ inline bool contains(V3 p) const {
-    return p.x >= 0.0F && p.y >= 0.0F && p.z >= 0.0F && p.x < static_cast<float>(g.nx) &&
-           p.y < static_cast<float>(g.ny) && p.z < static_cast<float>(g.nz);
-  }
-  __device__ inline V3 grid_coords(V3 p) const {
-    return scale3(sub3(p, mk3(g.ox, g.oy, g.oz)), inv_voxel);
-  }
-  __device__ static inline float axis_w(float f, int o) { return o == 0 ? 1.0F - f : f; }
-
-  // TrilinearSampler::tsdf + blend_tsdf.
-  __device__ bool tsdf(V3 p, bool require_all, float& out) const {
-    const V3 gc = grid_coords(p);
-    if (!contains(gc)) return false;
-    const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
-    const V3 fl = mk3(floorf(gg.x), floorf(gg.y), floorf(gg.z));
-    const int bx = static_cast<int>(fl.x), by = static_cast<int>(fl.y), bz = static_cast<int>(fl.z);
-    const V3 fr = sub3(gg, fl);
-    const bool interior = bx >= 0 && by >= 0 && bz >= 0 && bx + 1 < g.nx && by + 1 < g.ny &&
-                          bz + 1 < g.nz;
-    float acc = 0.0F, wsum = 0.0F;
-    int valid = 0;
-#pragma unroll
-    for (int oz = 0; oz <= 1; ++oz)
-#pragma unroll
-      for (int oy = 0; oy <= 1; ++oy)
-#pragma unroll
-        for (int ox = 0; ox <= 1; ++ox) {
-          const int x = bx + ox, y = by + oy, z = bz + oz;
-          if (!interior && (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz)) continue;
-          float t, w;
-          bool obs, fin;
-          IO::unpack(vox[g.flat(x, y, z)], t, w, obs, fin);
-          if (!obs || !fin) continue;
-          const float cw = axis_w(fr.x, ox) * axis_w(fr.y, oy) * axis_w(fr.z, oz);
-          acc += cw * t;
-          wsum += cw;
-          ++valid;
-        }
-    if (require_all && valid < 8) return false;
-    if (wsum <= 1.0e-6F) return false;
-    const float d = acc / wsum;
-    if (!isfinite(d)) return false;
-    out = d;
-    return true;
-  }
-
-  // blend_color.
-  __device__ bool color(V3 p, V3& out) const {
-    if (colors == nullptr) return false;
-    const V3 gc = grid_coords(p);
-    if (!contains(gc)) return false;
-    const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
-    const V3 fl = mk3(floorf(gg.x), floorf(gg.y), floorf(gg.z));
-    const int bx = static_cast<int>(fl.x), by = static_cast<int>(fl.y), bz = static_cast<int>(fl.z);
-    const V3 fr = sub3(gg, fl);
-    V3 acc = mk3(0.0F, 0.0F, 0.0F);
-    float wsum = 0.0F;
-    for (int oz = 0; oz <= 1; ++oz)
-      for (int oy = 0; oy <= 1; ++oy)
-        for (int ox = 0; ox <= 1; ++ox) {
-          const int x = bx + ox, y = by + oy, z = bz + oz;
-          if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
-          const float4 c = colors[g.flat(x, y, z)];
-          if (c.w <= 0.0F) continue;
-          const float cw = axis_w(fr.x, ox) * axis_w(fr.y, oy) * axis_w(fr.z, oz);
-          acc = add3(acc, scale3(mk3(c.x, c.y, c.z), cw));
-          wsum += cw;
-        }
-    if (wsum <= 1.0e-6F) return false;
-    out = div3(acc, wsum);
-    return true;
-  }
-
-  // central_normal (trilinear.hpp:246-269).
-  __device__ bool normal(V3 p, bool require_all, V3& out) const {
-    float gr[3];
-#pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-      const V3 off = scale3(mk3(axis == 0 ? 1.0F : 0.0F, axis == 1 ? 1.0F : 0.0F,
-                                axis == 2 ? 1.0F : 0.0F),
-                            g.voxel);
-      float plus, minus;
-      if (!tsdf(add3(p, off), require_all, plus)) return false;
-      if (!tsdf(sub3(p, off), require_all, minus)) return false;
-      gr[axis] = plus - minus;
-    }
-    const V3 dir = mk3(gr[0], gr[1], gr[2]);
-    const float len = norm3(dir);
-    if (len <= 0.0F) return false;
-    out = div3(dir, len);
-    return true;
-  }
-
-  // cell_normal + cell_gradient (volume_sampler.hpp:189-239).
-  __device__ bool cell_normal(V3 p, V3& out) const {
-    const V3 gc = grid_coords(p);
-    if (!contains(gc)) return false;
-    const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
-    const V3 fl = mk3(floorf(gg.x), floorf(gg.y), floorf(gg.z));
-    const int bx = static_cast<int>(fl.x), by = static_cast<int>(fl.y), bz = static_cast<int>(fl.z);
-    const V3 f = sub3(gg, fl);
-    float c[2][2][2];
-    for (int oz = 0; oz <= 1; ++oz)
-      for (int oy = 0; oy <= 1; ++oy)
-        for (int ox = 0; ox <= 1; ++ox) {
-          const int x = bx + ox, y = by + oy, z = bz + oz;
-          if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return normal(p, false, out);
-          float t, w;
-          bool obs, fin;
-          IO::unpack(vox[g.flat(x, y, z)], t, w, obs, fin);
-          if (!obs || !fin) return normal(p, false, out);
-          c[oz][oy][ox] = t;
-        }
-    auto lerp2 = [](float lo, float hi, float m) { return lo + (m * (hi - lo)); };
-    const float gx = lerp2(lerp2(c[0][0][1] - c[0][0][0], c[0][1][1] - c[0][1][0], f.y),
-                           lerp2(c[1][0][1] - c[1][0][0], c[1][1][1] - c[1][1][0], f.y), f.z);
-    const float gy = lerp2(lerp2(c[0][1][0] - c[0][0][0], c[0][1][1] - c[0][0][1], f.x),
-                           lerp2(c[1][1][0] - c[1][0][0], c[1][1][1] - c[1][0][1], f.x), f.z);
-    const float gz = lerp2(lerp2(c[1][0][0] - c[0][0][0], c[1][0][1] - c[0][0][1], f.x),
-                           lerp2(c[1][1][0] - c[0][1][0], c[1][1][1] - c[0][1][1], f.x), f.y);
-    const V3 grad = mk3(gx, gy, gz);
-    const float len = norm3(grad);
-    if (len <= 0.0F) return false;
-    out = div3(grad, len);
-    return true;
-  }
-};
-
 __device__ inline uint32_t pixel_from_color(V3 c) {
   auto to_byte = [](float v) { return static_cast<uint32_t>(fclamp_ref(v, 0.0F, 255.0F)); };
   return to_byte(c.x) | (to_byte(c.y) << 8U) | (to_byte(c.z) << 16U) | (0xFFU << 24U);
 }
 
 template <int Kind>
-__global__ void __launch_bounds__(256) raycast_kernel(Volume<Kind> vol, RayParams rp, Rigid pose_arg,
+__global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind> vol, RayParams rp, Rigid pose_arg,
                                                        IcpState* state, float4* __restrict__ pts,
                                                        float4* __restrict__ nrm,
                                                        uint32_t* __restrict__ col) {
@@ -237,7 +101,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(Volume<Kind> vol, RayParam
 template <int Kind>
 void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState* state,
               LevelBuffers& L) {
-  Volume<Kind> vol{static_cast<const typename VoxelIO<Kind>::Raw*>(ctx->voxels), ctx->colors,
+  Sampler<Kind> vol{static_cast<const typename VoxelIO<Kind>::Raw*>(ctx->voxels), ctx->colors,
                    ctx->grid, 1.0F / ctx->grid.voxel};
   const dim3 grid((rp.w + 15) / 16, (rp.h + 15) / 16);
   hipLaunchKernelGGL(raycast_kernel<Kind>, grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,

@@ -227,6 +227,9 @@ class Pipeline:
         N.check(N.hip().kf_create(C.byref(native), C.byref(ctx)), None)
         self._ctx = ctx
         self._native = native
+        # device inputs of enqueued work stay referenced until the next
+        # synchronisation, so torch cannot recycle their memory early
+        self._inflight: list = []
 
     @staticmethod
     def create(config: PipelineConfig) -> "Pipeline":
@@ -238,8 +241,9 @@ class Pipeline:
 
     def close(self) -> None:
         if getattr(self, "_ctx", None):
-            N.hip().kf_destroy(self._ctx)
+            N.hip().kf_destroy(self._ctx)  # synchronises the stream first
             self._ctx = None
+            self._inflight = []
 
     def __del__(self):
         try:
@@ -249,6 +253,9 @@ class Pipeline:
 
     def _check(self, status: int) -> None:
         N.check(status, self._ctx)
+
+    def hold(self, *tensors) -> None:
+        self._inflight.extend(t for t in tensors if t is not None)
 
     def reset(self) -> None:
         self._check(N.hip().kf_reset(self._ctx))
@@ -269,6 +276,7 @@ class Pipeline:
         w2c = np.asarray(world_to_camera, np.float64)[:3, :4].astype(np.float32).reshape(12)
         f.world_to_camera[:] = list(w2c)
         counts = (C.c_int64 * 2)() if count else None
+        self.hold(depth, color, normals)
         self._check(N.hip().kf_integrate(self._ctx, C.byref(f), counts))
         return (counts[0], counts[1]) if count else None
 
@@ -322,6 +330,7 @@ class Pipeline:
             self._torch.cuda.synchronize()
         opts = (icp or ProjectiveIcpOptions()).native()
         res = N.KfFrameResult() if report else None
+        self.hold(depth, color)
         self._check(N.hip().kf_process_frame(self._ctx, depth.data_ptr(),
                                              color.data_ptr() if color is not None else None, C.byref(opts),
                                              C.byref(res) if report else None))
@@ -342,6 +351,7 @@ class Pipeline:
 
     def synchronize(self) -> None:
         self._check(N.hip().kf_synchronize(self._ctx))
+        self._inflight.clear()
 
 
 def _wrap(torch, ptr: int, shape, dtype):
@@ -376,6 +386,7 @@ class PyramidSource:
 
     def build(self, raw_depth) -> int:
         self.pipeline._torch.cuda.synchronize()
+        self.pipeline.hold(raw_depth)
         n = C.c_int32()
         N.check(N.hip().kf_pyramid_build(self.pipeline.ctx, raw_depth.data_ptr(), C.byref(n)), self.pipeline.ctx)
         return n.value

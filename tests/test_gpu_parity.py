@@ -15,6 +15,10 @@ from kf_helpers import (INTR, RULE, STORE_KIND, H, W, f4, first_mismatch, host, 
 
 pytestmark = pytest.mark.gpu
 
+# 128^3 voxels of 2 cm span 2.56 m and hold the whole synthetic room; the
+# 1 cm volume of the 5-frame workload only reaches z = 0.78 m.
+ROOM_VOXEL = 0.02
+
 
 @pytest.fixture(scope="module")
 def K():
@@ -123,17 +127,17 @@ def test_integration_is_deterministic(K, scene, torch_gpu, ko):
             p.integrate(to_dev(torch_gpu, depth), ko.rigid_inverse(pose), color=to_dev(torch_gpu, rgba))
         outs.append(p.host_view())
         p.close()
-    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert same_bits(outs[0][0], outs[1][0]) and same_bits(outs[0][1], outs[1][1])
 
 
 # --- raycast -------------------------------------------------------------------------
 @pytest.mark.parametrize("store,color,cell", [("float", "float", False), ("bf16", "none", False),
                                               ("quantized", "float", True)])
 def test_raycast_bit_exact(K, ko, scene, torch_gpu, store, color, cell):
-    cfg = pipeline_config(K, store=store, color=color)
+    cfg = pipeline_config(K, store=store, color=color, voxel=ROOM_VOXEL)
     cfg.raycast.cell_gradient_normals = cell
     p = K.Pipeline(cfg)
-    vol = oracle_volume(ko, kind=STORE_KIND[store], color=color == "float")
+    vol = oracle_volume(ko, kind=STORE_KIND[store], color=color == "float", voxel=ROOM_VOXEL)
     for depth, rgba, pose in frames(scene, 2):
         w2c = ko.rigid_inverse(pose)
         ko.integrate(vol, 1, depth, INTR, w2c, ko.integration_options(1000.0), color=rgba)
@@ -155,8 +159,8 @@ def test_raycast_bit_exact(K, ko, scene, torch_gpu, store, color, cell):
 
 # --- ICP -------------------------------------------------------------------------------
 def tracking_setup(K, ko, scene, torch_gpu, n=2):
-    p = K.Pipeline(pipeline_config(K))
-    vol = oracle_volume(ko)
+    p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL))
+    vol = oracle_volume(ko, voxel=ROOM_VOXEL)
     for depth, rgba, pose in frames(scene, n):
         w2c = ko.rigid_inverse(pose)
         ko.integrate(vol, 1, depth, INTR, w2c, ko.integration_options(1000.0), color=rgba)
@@ -315,4 +319,44 @@ def test_frames_track_like_the_oracle(K, ko, scene, torch_gpu):
         print(f"frame {i}: vs oracle {ang:.2e} rad {dt:.2e} m; vs truth {gt_ang:.2e} rad {gt_dt:.2e} m")
         assert ang < 1e-4 and dt < 1e-4
         assert gt_dt < 0.01
+    p.close()
+
+
+# --- marching cubes -----------------------------------------------------------------
+def compare_mesh(gpu, ref):
+    pos, nrm, col, tri = ref
+    assert gpu.triangles.shape == tri.shape and np.array_equal(gpu.triangles, tri)
+    assert same_bits(gpu.positions, pos), first_mismatch(gpu.positions, pos)
+    assert same_bits(gpu.normals, nrm), first_mismatch(gpu.normals, nrm)
+    assert same_bits(gpu.colors, col), first_mismatch(gpu.colors, col)
+
+
+@pytest.mark.parametrize("store", ["float", "quantized", "bf16"])
+def test_mesh_bit_exact(K, ko, scene, torch_gpu, store):
+    """Triangle topology, welded vertex numbering, positions, normals and
+    colours identical to the CPU mesher (cell classification included)."""
+    p = K.Pipeline(pipeline_config(K, store=store, voxel=ROOM_VOXEL))
+    vol = oracle_volume(ko, kind=STORE_KIND[store], voxel=ROOM_VOXEL)
+    for depth, rgba, pose in frames(scene, 3):
+        w2c = ko.rigid_inverse(pose)
+        ko.integrate(vol, 1, depth, INTR, w2c, ko.integration_options(1000.0), color=rgba)
+        p.integrate(to_dev(torch_gpu, depth), w2c, color=to_dev(torch_gpu, rgba))
+    for min_weight in (0.0, 2.0):
+        ref = ko.mesh(vol, min_weight)
+        gpu = p.extract_mesh(min_weight)
+        print(f"{store} min_weight={min_weight}: {len(gpu.positions)} vertices, {len(gpu.triangles)} triangles")
+        assert len(ref[3]) > 1000
+        compare_mesh(gpu, ref)
+    p.close()
+
+
+def test_mesh_sphere_and_empty(K, ko, torch_gpu):
+    import test_oracle_known_answers as T
+
+    vol, _, _ = T.sphere_volume(ko)
+    cfg = K.PipelineConfig(volume=K.VolumeGeometry((T.SYN_RES,) * 3, T.SYN_VOXEL, T.SYN_ORIGIN, T.SYN_TRUNC))
+    p = K.Pipeline(cfg)
+    assert len(p.extract_mesh().triangles) == 0  # unobserved volume
+    p.upload_volume(vol.voxels, vol.colors)
+    compare_mesh(p.extract_mesh(), ko.mesh(vol))
     p.close()
