@@ -109,6 +109,7 @@ typedef struct kf_config {
   int32_t init_mode;
   int32_t icp_iterations[4]; /* per level; reference default 10/5/4 */
   int32_t stage_timing;      /* record HIP events per stage in kf_process_frame */
+  int32_t count_fused;       /* kf_process_frame accumulates fused-voxel counts */
 } kf_config;
 
 /* ProjectiveIcpOptions (icp_optimizer.hpp:82-102). */
@@ -181,7 +182,8 @@ typedef struct kf_stage_times {
   float pyramid_ms;
   float raycast_ms;
   float icp_ms;
-  float integrate_ms;
+  float integrate_ms;        /* prep + cull + integrate kernels */
+  float integrate_kernel_ms; /* the voxel-update kernel alone */
   float total_ms;
 } kf_stage_times;
 
@@ -273,6 +275,10 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
 /* Poses of all frames processed so far (host, 16 doubles each). */
 kf_status kf_trajectory(kf_context* ctx, double* poses, int64_t capacity, int64_t* count);
 kf_status kf_last_stage_times(kf_context* ctx, kf_stage_times* out);
+/* Voxels fused (counts[0]) and colour voxels fused (counts[1]) by every
+ * kf_process_frame since kf_create / kf_reset when cfg.count_fused is set
+ * (the exact N of the integration's algorithmic bytes).  Synchronises. */
+kf_status kf_fused_counts(kf_context* ctx, int64_t* counts);
 
 /* ---- volume access: Pipeline::observed_voxel_count / host_view
  * (pipeline.hpp:87-91). */

@@ -41,6 +41,7 @@ class KfConfig(C.Structure):
         ("cy", C.c_float),
         ("hash_buckets", C.c_int64), ("block_capacity", C.c_int64),
         ("init_mode", C.c_int32), ("icp_iterations", C.c_int32 * 4), ("stage_timing", C.c_int32),
+        ("count_fused", C.c_int32),
     ]
 
 
@@ -84,7 +85,7 @@ class KfDepthFrame(C.Structure):
 
 class KfStageTimes(C.Structure):
     _fields_ = [("pyramid_ms", C.c_float), ("raycast_ms", C.c_float), ("icp_ms", C.c_float),
-                ("integrate_ms", C.c_float), ("total_ms", C.c_float)]
+                ("integrate_ms", C.c_float), ("integrate_kernel_ms", C.c_float), ("total_ms", C.c_float)]
 
 
 class KfFrameResult(C.Structure):
@@ -114,6 +115,7 @@ HIP_SYMBOLS = [
     ("kf_process_frame", _i32, [_vp, _vp, _vp, C.POINTER(KfIcpOptions), C.POINTER(KfFrameResult)]),
     ("kf_trajectory", _i32, [_vp, _dp, _i64, C.POINTER(_i64)]),
     ("kf_last_stage_times", _i32, [_vp, C.POINTER(KfStageTimes)]),
+    ("kf_fused_counts", _i32, [_vp, C.POINTER(_i64)]),
     ("kf_observed_voxel_count", _i32, [_vp, C.POINTER(_i64)]),
     ("kf_volume_bytes", _i32, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     ("kf_volume_download", _i32, [_vp, _vp, _vp]),

@@ -280,6 +280,7 @@ void kf_config_defaults(kf_config* c) {
   c->icp_iterations[2] = 4;
   c->icp_iterations[3] = 4;
   c->stage_timing = 0;
+  c->count_fused = 0;
 }
 
 void kf_icp_options_defaults(kf_icp_options* o) {
@@ -357,7 +358,8 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
                       ((c.resolution[2] + kBrickZ - 1) / kBrickZ);
     ctx->bricks = dalloc<uint32_t>(nb);
     ctx->counters = dalloc<uint32_t>(4);
-    ctx->fused = dalloc<unsigned long long>(2);
+    ctx->fused = dalloc<unsigned long long>(4);
+    KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     ctx->partials = dalloc<float>(512 * 32);
     ctx->ticket = dalloc<unsigned int>(1);
     KF_HIP(hipMemsetAsync(ctx->ticket, 0, sizeof(unsigned int), ctx->stream));
@@ -422,6 +424,7 @@ kf_status kf_synchronize(kf_context* ctx) {
 kf_status kf_reset(kf_context* ctx) {
   return guard(ctx, [&] {
     launch_fill_voxels(ctx);
+    KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     KF_HIP(hipGetLastError());
     ctx->frames = 0;
@@ -576,14 +579,14 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
       hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
                          static_cast<int>(KF_INIT_PREVIOUS));
       for (int s = 2; s < 2 + 2 * ctx->cfg.levels && timing; ++s) record(ctx, s);
-      launch_integrate(ctx, frame, false, true);
+      launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
     } else {
       hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
                          ctx->cfg.init_mode == KF_INIT_CONSTANT_VELOCITY ? KF_INIT_CONSTANT_VELOCITY
                                                                        : KF_INIT_PREVIOUS);
       const DeviceIcpParams p = to_device_params(*opt);
       track_levels(ctx, p, timing);
-      launch_integrate(ctx, frame, false, true);
+      launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
     }
     hipLaunchKernelGGL(frame_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->trajectory,
                        static_cast<long long>(ctx->frames));
@@ -633,8 +636,20 @@ kf_status kf_last_stage_times(kf_context* ctx, kf_stage_times* out) {
       prev = 3 + 2 * l;
     }
     t.integrate_ms = ms(prev, 11);
+    t.integrate_kernel_ms = ms(12, 13);
     t.total_ms = ms(0, 11);
     *out = t;
+  });
+}
+
+kf_status kf_fused_counts(kf_context* ctx, int64_t* counts) {
+  return guard(ctx, [&] {
+    require(counts != nullptr, "kf_fused_counts: null argument");
+    unsigned long long host[4];
+    KF_HIP(hipMemcpyAsync(host, ctx->fused, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    counts[0] = static_cast<int64_t>(host[2]);
+    counts[1] = static_cast<int64_t>(host[3]);
   });
 }
 

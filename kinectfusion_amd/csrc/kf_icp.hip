@@ -6,7 +6,7 @@
 // icp_correspondence.cu:216-343) with its host epilogue
 // (src/kinectfusion/icp_optimizer.cpp:82-169).
 //
-// One kernel per Gauss-Newton / LM trial:
+// Two kernels per Gauss-Newton / LM trial:
 //  * every thread matches its pixels and keeps the 29 per-correspondence
 //    terms (21 J^T J, 6 J^T r, distance, r^2) in FP32 registers;
 //  * a wave reduces them with an xor butterfly, the four waves of a
@@ -14,9 +14,10 @@
 //    writes one partial row -- no float atomics, so the sums do not depend
 //    on scheduling;
 //  * the last workgroup to finish (ticket counter) adds the partial rows in
-//    block order with Kahan-compensated FP32, converts to FP64 and runs the
-//    trial: FP64 pivoted LDL^T solve, caps, Rodrigues increment and pose
-//    composition in FP64, convergence and failure latches.
+//    a fixed order with Kahan-compensated FP32;
+//  * a one-wave kernel converts the sums to FP64 and runs the trial: FP64
+//    pivoted LDL^T solve, caps, Rodrigues increment and pose composition in
+//    FP64, convergence and failure latches.
 // Once a level's status latches (converged or failed) the remaining trial
 // kernels return at once, so a converged level costs no further sweeps.  The
 // stability spectrum (FP64 cyclic Jacobi) is checked once per level in
@@ -133,24 +134,58 @@ __device__ inline bool last_block(unsigned int* ticket) {
   return is_last;
 }
 
-// Kahan-compensated FP32 sum of the partial rows in block order; thread t
-// owns column t.  Result lands in out[0..29] (out[29]: count bits).
+// Kahan-compensated FP32 sum of the partial rows.  Column c (29 sums and
+// the count) is split over 8 threads: part q adds blocks q, q + 8, ... with
+// Kahan, then thread c adds the 8 part sums (and their compensations) in
+// order q = 0..7.  The order is fixed, so the result does not depend on
+// scheduling; the loads of a part are independent and issue back to back.
 __device__ inline void combine_partials(const float* __restrict__ partial, int nblocks, float* out) {
-  const int t = threadIdx.x;
-  if (t < kSums) {
-    float sum = 0.0F, comp = 0.0F;
-    for (int b = 0; b < nblocks; ++b) {
-      const float v = __builtin_nontemporal_load(&partial[b * kPartialStride + t]);
-      const float y = v - comp;
-      const float s2 = sum + y;
-      comp = (s2 - sum) - y;
-      sum = s2;
+  __shared__ float part_sum[8][kPartialStride];
+  __shared__ float part_comp[8][kPartialStride];
+  __shared__ unsigned part_cnt[8];
+  const int col = threadIdx.x & 31, q = threadIdx.x >> 5;  // 256 threads: 32 columns x 8 parts
+  float sum = 0.0F, comp = 0.0F;
+  unsigned cnt = 0;
+  if (col < kSums) {
+    int b = q;
+    for (; b + 24 < nblocks; b += 32) {
+      float v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = partial[(b + 8 * u) * kPartialStride + col];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float y = v[u] - comp;
+        const float t = sum + y;
+        comp = (t - sum) - y;
+        sum = t;
+      }
     }
-    out[t] = sum;
-  } else if (t == kSums) {
-    unsigned int c = 0;
-    for (int b = 0; b < nblocks; ++b)
-      c += __float_as_uint(__builtin_nontemporal_load(&partial[b * kPartialStride + kSums]));
+    for (; b < nblocks; b += 8) {
+      const float y = partial[b * kPartialStride + col] - comp;
+      const float t = sum + y;
+      comp = (t - sum) - y;
+      sum = t;
+    }
+  } else if (col == kSums) {
+    for (int b = q; b < nblocks; b += 8) cnt += __float_as_uint(partial[b * kPartialStride + kSums]);
+  }
+  part_sum[q][col] = sum;
+  part_comp[q][col] = comp;
+  if (col == kSums) part_cnt[q] = cnt;
+  __syncthreads();
+  if (threadIdx.x < kSums) {
+    float s2 = 0.0F, c2 = 0.0F;
+    for (int k = 0; k < 8; ++k) {
+      // add the part's value net of its own compensation
+      const float y = (part_sum[k][threadIdx.x] - part_comp[k][threadIdx.x]) - c2;
+      const float t = s2 + y;
+      c2 = (t - s2) - y;
+      s2 = t;
+    }
+    out[threadIdx.x] = s2;
+  } else if (threadIdx.x == kSums) {
+    unsigned c = 0;
+    for (int k = 0; k < 8; ++k) c += part_cnt[k];
     out[kSums] = __uint_as_float(c);
   }
 }
@@ -269,7 +304,6 @@ __device__ void run_trial(IcpState* st, const DeviceIcpParams& p, const float* s
   double sums[kSums];
   for (int i = 0; i < kSums; ++i) sums[i] = static_cast<double>(sums32[i]);
   const long long count = static_cast<long long>(__float_as_uint(sums32[kSums]));
-  for (int i = 0; i < 30; ++i) st->sums32[i] = sums32[i];
   if (count < p.min_corr) {
     for (int i = 0; i < kSums; ++i) st->eq[i] = sums[i];
     st->eq_count = count;
@@ -332,9 +366,11 @@ __device__ void run_trial(IcpState* st, const DeviceIcpParams& p, const float* s
   if (accepted && ut < p.min_t && ang < p.min_r) st->status = 1;
 }
 
-__global__ void __launch_bounds__(256) icp_trial_kernel(Maps m, DeviceIcpParams p, IcpState* st,
-                                                         float* __restrict__ partial,
-                                                         unsigned int* __restrict__ ticket) {
+// Correspondence sweep of one trial: sums land in st->sums32.  The solve
+// runs in icp_solve_kernel so that this kernel stays small (high occupancy).
+__global__ void __launch_bounds__(256) icp_trial_sweep_kernel(Maps m, DeviceIcpParams p, IcpState* st,
+                                                               float* __restrict__ partial,
+                                                               unsigned int* __restrict__ ticket) {
   __shared__ float lds[4][kSums + 1];
   __shared__ float combined[kSums + 1];
   __shared__ float xf[24];
@@ -353,10 +389,17 @@ __global__ void __launch_bounds__(256) icp_trial_kernel(Maps m, DeviceIcpParams 
   if (!last_block(ticket)) return;
   combine_partials(partial, gridDim.x, combined);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    run_trial(st, p, combined);
-    *ticket = 0U;
-  }
+  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
+  if (threadIdx.x == 0) *ticket = 0U;
+}
+
+// The trial's FP64 solve and pose update (one thread).
+__global__ void __launch_bounds__(64) icp_solve_kernel(DeviceIcpParams p, IcpState* st) {
+  if (threadIdx.x != 0) return;
+  if (st->status != 0 || st->frame_failed) return;
+  float sums[kSums + 1];
+  for (int i = 0; i <= kSums; ++i) sums[i] = st->sums32[i];
+  run_trial(st, p, sums);
 }
 
 // Single sweep with explicit float transforms (kf_icp_reduce).
@@ -506,8 +549,9 @@ void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, i
   const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
   const int nb = blocks_for(lw * lh);
   for (int it = 0; it < iterations; ++it) {
-    hipLaunchKernelGGL(icp_trial_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
+    hipLaunchKernelGGL(icp_trial_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
                        ctx->partials, ctx->ticket);
+    hipLaunchKernelGGL(icp_solve_kernel, dim3(1), dim3(64), 0, ctx->stream, p, ctx->icp);
   }
   hipLaunchKernelGGL(icp_finish_level_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, p, level);
   KF_HIP(hipGetLastError());

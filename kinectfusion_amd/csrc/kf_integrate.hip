@@ -256,6 +256,11 @@ __global__ void zero_counters_kernel(uint32_t* c, unsigned long long* f) {
   if (threadIdx.x < 2) f[threadIdx.x] = 0ULL;
 }
 
+// running totals f[2..3] += per-call counts f[0..1]
+__global__ void accumulate_counts_kernel(unsigned long long* f) {
+  if (threadIdx.x < 2) f[2 + threadIdx.x] += f[threadIdx.x];
+}
+
 template <int Kind>
 __global__ void fill_voxels_kernel(typename VoxelIO<Kind>::Raw* v, float4* colors, size_t n) {
   const typename VoxelIO<Kind>::Raw fresh = VoxelIO<Kind>::fresh();
@@ -302,7 +307,8 @@ void launch_fill_voxels(kf_context* ctx) {
   KF_HIP(hipGetLastError());
 }
 
-void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state) {
+void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state,
+                      bool accumulate) {
   const kf_config& c = ctx->cfg;
   FrameParams f{};
   f.w2c = rigid_from12(fr.world_to_camera);
@@ -334,6 +340,8 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
                      ctx->tile_max, tiles_x, tiles_y, ctx->bricks, ctx->counters);
   const bool color = ctx->colors != nullptr;
   const uint32_t* cimg = fr.color;
+  const bool timing = c.stage_timing != 0 && from_state;
+  if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
   switch (c.voxel_store) {
     case KF_VOXEL_QUANTIZED:
       color ? launch_integrate_t<KF_VOXEL_QUANTIZED, true>(ctx, f, cimg, count)
@@ -348,6 +356,9 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
             : launch_integrate_t<KF_VOXEL_FLOAT, false>(ctx, f, cimg, count);
       break;
   }
+  if (timing) KF_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
+  if (count && accumulate)
+    hipLaunchKernelGGL(accumulate_counts_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
   KF_HIP(hipGetLastError());
 }
 

@@ -287,7 +287,8 @@ struct kf_context {
   int64_t traj_capacity = 0;
   int64_t frames = 0;
   // timing
-  hipEvent_t ev[12] = {};  // 0 start, 1 pyramid, 2.. per level (raycast, icp), 11 end
+  hipEvent_t ev[14] = {};  // 0 start, 1 pyramid, 2.. per level (raycast, icp), 11 end,
+                          // 12/13 around the voxel-update kernel
   kf_stage_times last_times{};
   bool have_times = false;
   // learned warm start (pose network), created on demand
@@ -310,7 +311,8 @@ void hip_check(hipError_t err, const char* what);
 // Launch helpers implemented per stage.
 void launch_bilateral(kf_context* ctx, const uint16_t* raw);
 void launch_level_maps(kf_context* ctx, int level);
-void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state);
+void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state,
+                      bool accumulate = false);
 void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
                     bool skip_if_failed);
 void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,

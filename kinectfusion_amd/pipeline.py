@@ -115,6 +115,7 @@ class PipelineConfig:
     init: str = "previous"                     # previous | constant-velocity | learned
     icp_iterations: tuple[int, ...] = (10, 5, 4)
     stage_timing: bool = False
+    count_fused: bool = False
 
     def native(self) -> N.KfConfig:
         if self.space != "hip":
@@ -151,6 +152,7 @@ class PipelineConfig:
         its = list(self.icp_iterations) + [4] * (4 - len(self.icp_iterations))
         c.icp_iterations[:] = its[:4]
         c.stage_timing = int(self.stage_timing)
+        c.count_fused = int(self.count_fused)
         return c
 
 
@@ -348,6 +350,11 @@ class Pipeline:
         t = N.KfStageTimes()
         self._check(N.hip().kf_last_stage_times(self._ctx, C.byref(t)))
         return t
+
+    def fused_counts(self) -> tuple[int, int]:
+        out = (C.c_int64 * 2)()
+        self._check(N.hip().kf_fused_counts(self._ctx, out))
+        return out[0], out[1]
 
     def synchronize(self) -> None:
         self._check(N.hip().kf_synchronize(self._ctx))

@@ -261,7 +261,7 @@ def test_reference_icp_cases_on_device(K, ko, torch_gpu, case):
 
 
 # --- whole frames -------------------------------------------------------------------
-def oracle_run(ko, scene, n, store=0, res=128, voxel=0.01, margin=0.5):
+def oracle_run(ko, scene, n, store=0, res=128, voxel=ROOM_VOXEL, margin=0.5):
     """Reconstruction::initialize + process_frame on the oracle: integrate
     frame 0 at the identity, then track coarse-to-fine from the previous pose
     and integrate when every level succeeds."""
@@ -300,10 +300,11 @@ def pose_error(a, b):
 
 
 def test_frames_track_like_the_oracle(K, ko, scene, torch_gpu):
-    """The 5-frame 128^3 workload end to end: per-frame pose within 1e-4 rad
-    and 1e-4 m of the FP64 oracle, and close to the ground truth."""
+    """Five frames end to end on a 128^3 volume holding the room: per-frame
+    pose within 1e-4 rad and 1e-4 m of the FP64 oracle, and within 1 cm of the
+    ground truth."""
     n = 5
-    p = K.Pipeline(pipeline_config(K))
+    p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL))
     for i in range(n):
         depth, rgba, _ = scene.frame(i)
         res = p.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba))
@@ -360,3 +361,17 @@ def test_mesh_sphere_and_empty(K, ko, torch_gpu):
     p.upload_volume(vol.voxels, vol.colors)
     compare_mesh(p.extract_mesh(), ko.mesh(vol))
     p.close()
+
+
+def test_small_workload_runs_and_is_deterministic(K, scene, torch_gpu):
+    """The 5-frame 128^3 / 1 cm workload (its volume reaches z = 0.78 m only):
+    every frame runs, and two runs give byte-identical poses and volumes."""
+    outs = []
+    for _ in range(2):
+        p = K.Pipeline(pipeline_config(K))
+        for i in range(5):
+            depth, rgba, _ = scene.frame(i)
+            p.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba))
+        outs.append((p.trajectory(), p.host_view()[0]))
+        p.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
