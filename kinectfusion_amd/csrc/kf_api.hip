@@ -357,6 +357,13 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
                       ((c.resolution[1] + kBrickY - 1) / kBrickY) *
                       ((c.resolution[2] + kBrickZ - 1) / kBrickZ);
     ctx->bricks = dalloc<uint32_t>(nb);
+    {
+      const BlockMap bm = block_map(ctx->grid);
+      ctx->block_words = (static_cast<size_t>(bm.bx) * bm.by * bm.bz + 31) / 32;
+      ctx->obs_bits = dalloc<uint32_t>(ctx->block_words);
+      ctx->skip_bits = dalloc<uint32_t>(ctx->block_words);
+      KF_HIP(hipMemsetAsync(ctx->obs_bits, 0, ctx->block_words * 4, ctx->stream));
+    }
     ctx->counters = dalloc<uint32_t>(4);
     ctx->fused = dalloc<unsigned long long>(4);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
@@ -370,6 +377,7 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->trajectory = dalloc<double>(ctx->traj_capacity * 16);
     for (auto& e : ctx->ev) KF_HIP(hipEventCreate(&e));
     launch_fill_voxels(ctx);
+    launch_skip_map(ctx);
     hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     KF_HIP(hipStreamSynchronize(ctx->stream));
   });
@@ -400,6 +408,8 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->pix);
   dfree(ctx->tile_max);
   dfree(ctx->bricks);
+  dfree(ctx->obs_bits);
+  dfree(ctx->skip_bits);
   dfree(ctx->counters);
   dfree(ctx->fused);
   dfree(ctx->partials);
@@ -424,6 +434,8 @@ kf_status kf_synchronize(kf_context* ctx) {
 kf_status kf_reset(kf_context* ctx) {
   return guard(ctx, [&] {
     launch_fill_voxels(ctx);
+    KF_HIP(hipMemsetAsync(ctx->obs_bits, 0, ctx->block_words * 4, ctx->stream));
+    launch_skip_map(ctx);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     KF_HIP(hipGetLastError());
@@ -706,6 +718,7 @@ kf_status kf_volume_upload(kf_context* ctx, const void* voxels, const float* col
     if (colors != nullptr && ctx->colors != nullptr)
       KF_HIP(hipMemcpyAsync(ctx->colors, colors, ctx->n_voxels * sizeof(float4), hipMemcpyHostToDevice,
                             ctx->stream));
+    launch_rebuild_observed(ctx);
     KF_HIP(hipStreamSynchronize(ctx->stream));
   });
 }

@@ -195,55 +195,96 @@ __device__ inline void combine_partials(const float* __restrict__ partial, int n
 // (icp_optimizer.cpp:391): a pivot with |d| <= DBL_MIN contributes a zero
 // solution component, so a damped semidefinite system still solves; the
 // solve fails only on a non-finite solution.
+__device__ inline void swap_d(double& a, double& b) {
+  const double t = a;
+  a = b;
+  b = t;
+}
+
 __device__ bool solve6(const double* eq, int damping_mode, double scale, double* x) {
+  // Every index below is a compile-time constant after unrolling (the pivot
+  // swap is predicated), so the factorisation lives in registers.
   double a[6][6];
-  int e = 0;
-  for (int r = 0; r < 6; ++r)
-    for (int c = r; c < 6; ++c, ++e) {
-      a[r][c] = eq[e];
-      a[c][r] = eq[e];
-    }
+  {
+    int e = 0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = r; c < 6; ++c, ++e) {
+        a[r][c] = eq[e];
+        a[c][r] = eq[e];
+      }
+  }
+#pragma unroll
   for (int r = 0; r < 6; ++r) {
     if (damping_mode == 1) a[r][r] += scale;
     else if (damping_mode == 2) a[r][r] += scale * a[r][r];
   }
   int perm[6] = {0, 1, 2, 3, 4, 5};
   double d[6];
+#pragma unroll
   for (int k = 0; k < 6; ++k) {
     int piv = k;
-    for (int i = k + 1; i < 6; ++i)
-      if (fabs(a[i][i]) > fabs(a[piv][piv])) piv = i;
-    if (piv != k) {
-      for (int c = 0; c < 6; ++c) { const double t = a[k][c]; a[k][c] = a[piv][c]; a[piv][c] = t; }
-      for (int r = 0; r < 6; ++r) { const double t = a[r][k]; a[r][k] = a[r][piv]; a[r][piv] = t; }
-      const int tp = perm[k]; perm[k] = perm[piv]; perm[piv] = tp;
+    double best = fabs(a[k][k]);
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      const double v = fabs(a[i][i]);
+      if (v > best) {
+        best = v;
+        piv = i;
+      }
+    }
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      if (piv == i) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) swap_d(a[k][c], a[i][c]);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) swap_d(a[r][k], a[r][i]);
+        const int tp = perm[k];
+        perm[k] = perm[i];
+        perm[i] = tp;
+      }
     }
     d[k] = a[k][k];
     const bool zero = fabs(d[k]) <= 2.2250738585072014e-308;
+#pragma unroll
     for (int i = k + 1; i < 6; ++i) a[i][k] = zero ? 0.0 : a[i][k] / d[k];
+#pragma unroll
     for (int i = k + 1; i < 6; ++i)
-      for (int j = k + 1; j <= i; ++j) {
-        a[i][j] -= a[i][k] * d[k] * a[j][k];
-        a[j][i] = a[i][j];
+#pragma unroll
+      for (int jj = k + 1; jj <= i; ++jj) {
+        a[i][jj] -= a[i][k] * d[k] * a[jj][k];
+        a[jj][i] = a[i][jj];
       }
   }
   double y[6];
+#pragma unroll
   for (int i = 0; i < 6; ++i) {
     double s = eq[21 + perm[i]];
+#pragma unroll
     for (int q = 0; q < i; ++q) s -= a[i][q] * y[q];
     y[i] = s;
   }
+#pragma unroll
   for (int i = 0; i < 6; ++i) y[i] = fabs(d[i]) <= 2.2250738585072014e-308 ? 0.0 : y[i] / d[i];
   double z[6];
+#pragma unroll
   for (int i = 5; i >= 0; --i) {
     double s = y[i];
+#pragma unroll
     for (int q = i + 1; q < 6; ++q) s -= a[q][i] * z[q];
     z[i] = s;
   }
-  for (int i = 0; i < 6; ++i) x[perm[i]] = z[i];
+#pragma unroll
   for (int i = 0; i < 6; ++i)
-    if (!isfinite(x[i])) return false;
-  return true;
+#pragma unroll
+    for (int jj = 0; jj < 6; ++jj)
+      if (perm[i] == jj) x[jj] = z[i];
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) ok = ok && isfinite(x[i]);
+  return ok;
 }
 
 // pose <- exp([w]) pose in FP64 (Rodrigues; left-multiplied world-frame
@@ -426,23 +467,33 @@ __device__ void sym_eig6(const double* eq, double* ev) {
       a[r][c] = eq[e];
       a[c][r] = eq[e];
     }
+  // Sweeps stop once the off-diagonal mass is below (1e-16)^2 of the
+  // diagonal's: the eigenvalues are then exact to ~1e-16 of the norm.
   for (int sweep = 0; sweep < 32; ++sweep) {
-    double off = 0.0;
+    double off = 0.0, diag = 0.0;
+#pragma unroll
+    for (int d = 0; d < 6; ++d) diag += a[d][d] * a[d][d];
+#pragma unroll
     for (int pp = 0; pp < 6; ++pp)
+#pragma unroll
       for (int q = pp + 1; q < 6; ++q) off += a[pp][q] * a[pp][q];
-    if (off == 0.0) break;
+    if (off <= 1.0e-32 * diag) break;
+#pragma unroll
     for (int pp = 0; pp < 6; ++pp)
+#pragma unroll
       for (int q = pp + 1; q < 6; ++q) {
         if (a[pp][q] == 0.0) continue;
         const double theta = (a[q][q] - a[pp][pp]) / (2.0 * a[pp][q]);
         const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
         const double c = 1.0 / sqrt(t * t + 1.0);
         const double s = t * c;
+#pragma unroll
         for (int k2 = 0; k2 < 6; ++k2) {
           const double akp = a[k2][pp], akq = a[k2][q];
           a[k2][pp] = c * akp - s * akq;
           a[k2][q] = s * akp + c * akq;
         }
+#pragma unroll
         for (int k2 = 0; k2 < 6; ++k2) {
           const double apk = a[pp][k2], aqk = a[q][k2];
           a[pp][k2] = c * apk - s * aqk;

@@ -178,27 +178,16 @@ __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
 }
 
 // 3. integrate the listed bricks.
+// The voxel updates of one (x, y) column of a brick; true when any fused.
 template <int Kind, bool Color, bool Count>
-__global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
-                                                         const float4* __restrict__ pix,
-                                                         const uint32_t* __restrict__ color_img,
-                                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
-                                                         float4* __restrict__ colors,
-                                                         const uint32_t* __restrict__ list,
-                                                         const uint32_t* __restrict__ count,
-                                                         unsigned long long* __restrict__ fused) {
+__device__ inline bool integrate_column(const Grid& g, const FrameParams& f, const Rigid& w2c,
+                                        const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
+                                        typename VoxelIO<Kind>::Raw* __restrict__ vox,
+                                        float4* __restrict__ colors, int x, int y, int bz,
+                                        unsigned int& my_fused, unsigned int& my_color) {
   using IO = VoxelIO<Kind>;
-  Rigid w2c;
-  if (!block_w2c(f, w2c)) return;
-  const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
-  const uint32_t n = *count;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  unsigned int my_fused = 0, my_color = 0;
-  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
-    const uint32_t b = list[li];
-    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
-    const int x = bx * kBrickX + tx, y = by * kBrickY + ty;
-    if (x >= g.nx || y >= g.ny) continue;
+  bool any = false;
+  {
     for (int zz = 0; zz < kBrickZ; ++zz) {
       const int z = bz * kBrickZ + zz;
       if (z >= g.nz) break;
@@ -223,6 +212,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
       const float avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
       const float tw = fmin_ref(w0 + w, f.max_weight);
       vox[vi] = IO::pack(avg, tw);
+      any = true;
       if (Count) ++my_fused;
       if (Color) {
         if (color_img != nullptr && sd <= p.w * 0.5F) {
@@ -239,6 +229,45 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
       }
     }
   }
+  return any;
+}
+
+template <int Kind, bool Color, bool Count>
+__global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
+                                                         const float4* __restrict__ pix,
+                                                         const uint32_t* __restrict__ color_img,
+                                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
+                                                         float4* __restrict__ colors,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ count,
+                                                         unsigned long long* __restrict__ fused,
+                                                         uint32_t* __restrict__ obs_bits, BlockMap bm) {
+  Rigid w2c;
+  if (!block_w2c(f, w2c)) return;
+  const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
+  const uint32_t n = *count;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  unsigned int my_fused = 0, my_color = 0;
+  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+    const uint32_t b = list[li];
+    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
+    const int x = bx * kBrickX + tx, y = by * kBrickY + ty;
+    bool any = false;
+    // The brick's rows of 64 voxels span 8 occupancy blocks along x and one
+    // along y and z: one ballot per wave marks them (lane 0 publishes).
+    if (x < g.nx && y < g.ny) any = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors,
+                                                                           x, y, bz, my_fused, my_color);
+    const unsigned long long mask = __ballot(any);
+    if (tx == 0 && mask != 0ULL) {
+      const int blk_y = (by * kBrickY) >> 3, blk_z = (bz * kBrickZ) >> 3;
+      for (int j = 0; j < 8; ++j) {
+        const int blk_x = bx * 8 + j;
+        if (((mask >> (8 * j)) & 0xFFULL) == 0ULL || blk_x >= bm.bx) continue;
+        const int fb = bm.flat(blk_x, blk_y, blk_z);
+        atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
+      }
+    }
+  }
   if (Count) {
     for (int off = 32; off > 0; off >>= 1) {
       my_fused += __shfl_xor(my_fused, off);
@@ -251,6 +280,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
   }
 }
 
+
 __global__ void zero_counters_kernel(uint32_t* c, unsigned long long* f) {
   if (threadIdx.x < 4) c[threadIdx.x] = 0U;
   if (threadIdx.x < 2) f[threadIdx.x] = 0ULL;
@@ -259,6 +289,63 @@ __global__ void zero_counters_kernel(uint32_t* c, unsigned long long* f) {
 // running totals f[2..3] += per-call counts f[0..1]
 __global__ void accumulate_counts_kernel(unsigned long long* f) {
   if (threadIdx.x < 2) f[2 + threadIdx.x] += f[threadIdx.x];
+}
+
+// skip bit of a block = the block and its 26 neighbours hold no observed
+// voxel.  One thread per block; a wave's 64 consecutive blocks form two words.
+__global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restrict__ obs, uint32_t* __restrict__ skip,
+                                                        BlockMap bm, int nblocks) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  bool empty = false;
+  if (b < nblocks) {
+    const int x = b % bm.bx, y = (b / bm.bx) % bm.by, z = b / (bm.bx * bm.by);
+    empty = true;
+    for (int dz = -1; dz <= 1 && empty; ++dz)
+      for (int dy = -1; dy <= 1 && empty; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int xx = x + dx, yy = y + dy, zz = z + dz;
+          if (xx < 0 || yy < 0 || zz < 0 || xx >= bm.bx || yy >= bm.by || zz >= bm.bz) continue;
+          if (bm.bit(obs, bm.flat(xx, yy, zz))) {
+            empty = false;
+            break;
+          }
+        }
+  }
+  const unsigned long long m = __ballot(empty);
+  const int lane = threadIdx.x & 63;
+  const int word0 = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 5;
+  const int nwords = (nblocks + 31) >> 5;
+  if (lane == 0 && word0 < nwords) skip[word0] = static_cast<uint32_t>(m);
+  if (lane == 32 && word0 + 1 < nwords) skip[word0 + 1] = static_cast<uint32_t>(m >> 32);
+}
+
+// observed bits from the voxel array (after an upload): one thread per block.
+template <int Kind>
+__global__ void __launch_bounds__(256) rebuild_observed_kernel(const typename VoxelIO<Kind>::Raw* __restrict__ v,
+                                                                Grid g, uint32_t* __restrict__ obs, BlockMap bm,
+                                                                int nblocks) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  bool any = false;
+  if (b < nblocks) {
+    const int bx = b % bm.bx, by = (b / bm.bx) % bm.by, bz = b / (bm.bx * bm.by);
+    for (int z = bz * 8; z < min(bz * 8 + 8, g.nz) && !any; ++z)
+      for (int y = by * 8; y < min(by * 8 + 8, g.ny) && !any; ++y)
+        for (int x = bx * 8; x < min(bx * 8 + 8, g.nx); ++x) {
+          float t, w;
+          bool o, fin;
+          VoxelIO<Kind>::unpack(v[g.flat(x, y, z)], t, w, o, fin);
+          if (o) {
+            any = true;
+            break;
+          }
+        }
+  }
+  const unsigned long long m = __ballot(any);
+  const int lane = threadIdx.x & 63;
+  const int word0 = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 5;
+  const int nwords = (nblocks + 31) >> 5;
+  if (lane == 0 && word0 < nwords) obs[word0] = static_cast<uint32_t>(m);
+  if (lane == 32 && word0 + 1 < nwords) obs[word0 + 1] = static_cast<uint32_t>(m >> 32);
 }
 
 template <int Kind>
@@ -278,11 +365,13 @@ void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* c
   if (count) {
     hipLaunchKernelGGL((integrate_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
-                       ctx->colors, ctx->bricks, ctx->counters, ctx->fused);
+                       ctx->colors, ctx->bricks, ctx->counters, ctx->fused, ctx->obs_bits,
+                       block_map(ctx->grid));
   } else {
     hipLaunchKernelGGL((integrate_kernel<Kind, Color, false>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
-                       ctx->colors, ctx->bricks, ctx->counters, ctx->fused);
+                       ctx->colors, ctx->bricks, ctx->counters, ctx->fused, ctx->obs_bits,
+                       block_map(ctx->grid));
   }
 }
 
@@ -305,6 +394,36 @@ void launch_fill_voxels(kf_context* ctx) {
       break;
   }
   KF_HIP(hipGetLastError());
+}
+
+void launch_skip_map(kf_context* ctx) {
+  const BlockMap bm = block_map(ctx->grid);
+  const int nblocks = bm.bx * bm.by * bm.bz;
+  hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
+                     ctx->skip_bits, bm, nblocks);
+  KF_HIP(hipGetLastError());
+}
+
+void launch_rebuild_observed(kf_context* ctx) {
+  const BlockMap bm = block_map(ctx->grid);
+  const int nblocks = bm.bx * bm.by * bm.bz;
+  const dim3 grid((nblocks + 255) / 256);
+  switch (ctx->cfg.voxel_store) {
+    case KF_VOXEL_QUANTIZED:
+      hipLaunchKernelGGL(rebuild_observed_kernel<KF_VOXEL_QUANTIZED>, grid, dim3(256), 0, ctx->stream,
+                         static_cast<const uint32_t*>(ctx->voxels), ctx->grid, ctx->obs_bits, bm, nblocks);
+      break;
+    case KF_VOXEL_BF16:
+      hipLaunchKernelGGL(rebuild_observed_kernel<KF_VOXEL_BF16>, grid, dim3(256), 0, ctx->stream,
+                         static_cast<const uint32_t*>(ctx->voxels), ctx->grid, ctx->obs_bits, bm, nblocks);
+      break;
+    default:
+      hipLaunchKernelGGL(rebuild_observed_kernel<KF_VOXEL_FLOAT>, grid, dim3(256), 0, ctx->stream,
+                         static_cast<const float2*>(ctx->voxels), ctx->grid, ctx->obs_bits, bm, nblocks);
+      break;
+  }
+  KF_HIP(hipGetLastError());
+  launch_skip_map(ctx);
 }
 
 void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state,
@@ -357,6 +476,7 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
       break;
   }
   if (timing) KF_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
+  launch_skip_map(ctx);
   if (count && accumulate)
     hipLaunchKernelGGL(accumulate_counts_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
   KF_HIP(hipGetLastError());

@@ -276,6 +276,11 @@ struct kf_context {
   float4* pix = nullptr;          // per pixel: depth m, lambda, weight, valid
   float* tile_max = nullptr;      // per 16x16 tile max usable depth
   uint32_t* bricks = nullptr;     // active brick list
+  // one bit per 8^3 voxel block: any voxel observed / block and its 26
+  // neighbours all unobserved (the raycaster skips those)
+  uint32_t* obs_bits = nullptr;
+  uint32_t* skip_bits = nullptr;
+  size_t block_words = 0;
   uint32_t* counters = nullptr;   // [0] brick count, [1..2] fused counts (u64 pair as 2x u32), spare
   unsigned long long* fused = nullptr;
   // icp scratch
@@ -324,6 +329,18 @@ void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* l
                             float min_dot);
 void launch_icp_begin_level(kf_context* ctx, int level_index);
 void launch_fill_voxels(kf_context* ctx);
+void launch_skip_map(kf_context* ctx);      // skip bits from observed bits
+void launch_rebuild_observed(kf_context* ctx);  // observed bits from the voxels
+
+// Block grid of the occupancy bitmaps: 8^3 voxels per block, x fastest.
+struct BlockMap {
+  int bx, by, bz;  // blocks per axis
+  __host__ __device__ inline int flat(int x, int y, int z) const { return (z * by + y) * bx + x; }
+  __device__ inline bool bit(const uint32_t* bits, int f) const { return (bits[f >> 5] >> (f & 31)) & 1U; }
+};
+inline BlockMap block_map(const Grid& g) {
+  return BlockMap{(g.nx + 7) / 8, (g.ny + 7) / 8, (g.nz + 7) / 8};
+}
 DeviceIcpParams to_device_params(const kf_icp_options& o);
 
 }  // namespace kf

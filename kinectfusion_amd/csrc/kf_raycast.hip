@@ -26,11 +26,24 @@ __device__ inline uint32_t pixel_from_color(V3 c) {
   return to_byte(c.x) | (to_byte(c.y) << 8U) | (to_byte(c.z) << 16U) | (0xFFU << 24U);
 }
 
+// Exit parameter (ray length) of the ray gc(len) = a + b len from the box
+// [lo, hi) along each axis, in FP64.
+__device__ __forceinline__ double box_exit(const double (&a)[3], const double (&b)[3], const double (&lo)[3],
+                                           const double (&hi)[3]) {
+  double t = 1.0e300;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    if (b[k] > 1.0e-12) t = fmin(t, (hi[k] - a[k]) / b[k]);
+    else if (b[k] < -1.0e-12) t = fmin(t, (lo[k] - a[k]) / b[k]);
+  }
+  return t;
+}
+
 template <int Kind>
 __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind> vol, RayParams rp, Rigid pose_arg,
                                                        IcpState* state, float4* __restrict__ pts,
-                                                       float4* __restrict__ nrm,
-                                                       uint32_t* __restrict__ col) {
+                                                       float4* __restrict__ nrm, uint32_t* __restrict__ col,
+                                                       const uint32_t* __restrict__ skip_bits, BlockMap bm) {
   Rigid pose = pose_arg;
   if (state != nullptr) {
     if (state->frame_failed) return;
@@ -51,8 +64,62 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind> vol, RayPara
     float prev_t = 0.0F;
     V3 prev_p = mk3(0.0F, 0.0F, 0.0F), surface = prev_p;
     float len = rp.min_depth;
+    // Samples that provably return no TSDF are stepped over without touching
+    // the volume, reproducing the reference's float sequence of ray lengths:
+    // before the ray enters the volume box, after it has left it (it cannot
+    // come back), and inside 8^3 blocks whose 3x3x3 block neighbourhood holds
+    // no observed voxel.  The output is identical to marching every sample.
+    const double inv = 1.0 / static_cast<double>(vol.g.voxel);
+    const double ga[3] = {(static_cast<double>(pose.t.x) - vol.g.ox) * inv,
+                          (static_cast<double>(pose.t.y) - vol.g.oy) * inv,
+                          (static_cast<double>(pose.t.z) - vol.g.oz) * inv};
+    const double gb[3] = {static_cast<double>(dir.x) * inv, static_cast<double>(dir.y) * inv,
+                          static_cast<double>(dir.z) * inv};
+    const double dims[3] = {static_cast<double>(vol.g.nx), static_cast<double>(vol.g.ny),
+                            static_cast<double>(vol.g.nz)};
+    double t_in = -1.0e300, t_out = 1.0e300, slope_in = 1.0, slope_out = 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (fabs(gb[k]) <= 1.0e-12) {
+        if (ga[k] < 0.0 || ga[k] > dims[k]) t_out = -1.0e300;  // parallel and outside
+        continue;
+      }
+      double t0 = -ga[k] / gb[k], t1 = (dims[k] - ga[k]) / gb[k];
+      if (t0 > t1) {
+        const double tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+      }
+      if (t0 > t_in) {
+        t_in = t0;
+        slope_in = fabs(gb[k]);
+      }
+      if (t1 < t_out) {
+        t_out = t1;
+        slope_out = fabs(gb[k]);
+      }
+    }
+    // margins of 0.01 voxel along the deciding axis absorb the rounding of
+    // the float grid coordinates the reference compares
+    const float len_stop = t_out < t_in ? -1.0F : static_cast<float>(fmin(t_out + 0.01 / slope_out, 1.0e30));
+    const double len_enter = t_in - 0.01 / slope_in;
+    while (static_cast<double>(len) < len_enter && len <= rp.max_depth) len += base_step;
     while (len <= rp.max_depth) {
+      if (len > len_stop) break;
       const V3 point = add3(pose.t, scale3(dir, len));
+      const V3 gc = vol.grid_coords(point);
+      if (vol.contains(gc)) {
+        const int bx = static_cast<int>(gc.x) >> 3, by = static_cast<int>(gc.y) >> 3, bz = static_cast<int>(gc.z) >> 3;
+        if (bm.bit(skip_bits, bm.flat(bx, by, bz))) {
+          const double lo[3] = {8.0 * bx, 8.0 * by, 8.0 * bz};
+          const double hi[3] = {lo[0] + 8.0, lo[1] + 8.0, lo[2] + 8.0};
+          const double t_blk = box_exit(ga, gb, lo, hi);
+          have_prev = false;
+          len += base_step;  // this sample, then every sample still in the block
+          while (static_cast<double>(len) <= t_blk && len <= rp.max_depth) len += base_step;
+          continue;
+        }
+      }
       float t;
       if (!vol.tsdf(point, rp.require_all != 0, t)) {
         have_prev = false;
@@ -105,7 +172,7 @@ void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState*
                    ctx->grid, 1.0F / ctx->grid.voxel};
   const dim3 grid((rp.w + 15) / 16, (rp.h + 15) / 16);
   hipLaunchKernelGGL(raycast_kernel<Kind>, grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,
-                     L.model_pts, L.model_nrm, L.model_col);
+                     L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, block_map(ctx->grid));
 }
 
 }  // namespace

@@ -76,8 +76,11 @@ struct Sampler {
     const V3 fr = sub3(gg, fl);
     V3 acc = mk3(0.0F, 0.0F, 0.0F);
     float wsum = 0.0F;
+#pragma unroll
     for (int oz = 0; oz <= 1; ++oz)
+#pragma unroll
       for (int oy = 0; oy <= 1; ++oy)
+#pragma unroll
         for (int ox = 0; ox <= 1; ++ox) {
           const int x = bx + ox, y = by + oy, z = bz + oz;
           if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
@@ -121,17 +124,26 @@ struct Sampler {
     const int bx = static_cast<int>(fl.x), by = static_cast<int>(fl.y), bz = static_cast<int>(fl.z);
     const V3 f = sub3(gg, fl);
     float c[2][2][2];
+    bool complete = true;
+#pragma unroll
     for (int oz = 0; oz <= 1; ++oz)
+#pragma unroll
       for (int oy = 0; oy <= 1; ++oy)
+#pragma unroll
         for (int ox = 0; ox <= 1; ++ox) {
           const int x = bx + ox, y = by + oy, z = bz + oz;
-          if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) return normal(p, false, out);
+          c[oz][oy][ox] = 0.0F;
+          if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) {
+            complete = false;
+            continue;
+          }
           float t, w;
           bool obs, fin;
           IO::unpack(vox[g.flat(x, y, z)], t, w, obs, fin);
-          if (!obs || !fin) return normal(p, false, out);
+          if (!obs || !fin) complete = false;
           c[oz][oy][ox] = t;
         }
+    if (!complete) return normal(p, false, out);
     auto lerp2 = [](float lo, float hi, float m) { return lo + (m * (hi - lo)); };
     const float gx = lerp2(lerp2(c[0][0][1] - c[0][0][0], c[0][1][1] - c[0][1][0], f.y),
                            lerp2(c[1][0][1] - c[1][0][0], c[1][1][1] - c[1][1][0], f.y), f.z);

@@ -152,7 +152,7 @@ extern "C" {
 
 // kind 0: "room"  -- box room 1.2 m wide (x in [-0.6, 0.6], y in [-0.5, 0.5],
 //                    z in [-0.4, 1.8]) with 6 spheres of radius U(0.05, 0.2) m
-//                    whose centres lie 0.5-1.2 m in front of the first camera.
+//                    whose surfaces lie 0.5-1.2 m in front of the first camera.
 // kind 1: "hall"  -- 6 m x 3 m x 4 m room (x, y, z) with 20 boxes and 20
 //                    spheres; the camera pans and walks along x.
 void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, double fx, double fy,
@@ -174,7 +174,8 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
     for (int i = 0; i < 6; ++i) {
       Sphere s{};
       s.r = rng.uniform(0.05, 0.2);
-      const double dist = rng.uniform(0.5, 1.2);
+      // the sphere's surface lies 0.5-1.2 m from the first camera
+      const double dist = rng.uniform(0.5, 1.2) + s.r;
       const double u = rng.uniform(0.15, 0.85) * width, v = rng.uniform(0.15, 0.85) * height;
       D3 d{(u - cx) / fx, (v - cy) / fy, 1.0};
       d = d * (1.0 / std::sqrt(dot(d, d)));
@@ -188,7 +189,9 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
     // per-frame translation <= sqrt(3) * 0.0055 m, rotation <= 3 * 0.16 deg
     for (int a = 0; a < 3; ++a) {
       sc->t_freq[a] = rng.uniform(0.03, 0.06);
-      sc->t_amp[a] = 0.0055 / sc->t_freq[a] * rng.uniform(0.6, 1.0);
+      // the forward swing is kept small so surfaces stay beyond the 0.4 m
+      // minimum depth
+      sc->t_amp[a] = 0.0055 / sc->t_freq[a] * rng.uniform(0.6, 1.0) * (a == 2 ? 0.3 : 1.0);
       sc->t_phase[a] = rng.uniform(0.0, 6.283185307179586);
       sc->r_freq[a] = rng.uniform(0.03, 0.06);
       sc->r_amp[a] = (0.16 * 3.141592653589793 / 180.0) / sc->r_freq[a] * rng.uniform(0.6, 1.0);

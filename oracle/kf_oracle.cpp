@@ -789,11 +789,14 @@ void kfo_sym_eig6(const double* a_in, double* evals) {
   double a[6][6];
   for (int r = 0; r < 6; ++r)
     for (int c = 0; c < 6; ++c) a[r][c] = a_in[r * 6 + c];
+  // Sweeps stop once the off-diagonal mass is below (1e-16)^2 of the
+  // diagonal's: the eigenvalues are then exact to ~1e-16 of the norm.
   for (int sweep = 0; sweep < 32; ++sweep) {
-    double off = 0.0;
+    double off = 0.0, diag = 0.0;
+    for (int d = 0; d < 6; ++d) diag += a[d][d] * a[d][d];
     for (int p = 0; p < 6; ++p)
       for (int q = p + 1; q < 6; ++q) off += a[p][q] * a[p][q];
-    if (off == 0.0) break;
+    if (off <= 1.0e-32 * diag) break;
     for (int p = 0; p < 6; ++p)
       for (int q = p + 1; q < 6; ++q) {
         if (a[p][q] == 0.0) continue;

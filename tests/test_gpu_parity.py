@@ -157,6 +157,47 @@ def test_raycast_bit_exact(K, ko, scene, torch_gpu, store, color, cell):
     p.close()
 
 
+@pytest.mark.parametrize("store", ["float", "bf16"])
+def test_raycast_through_holes_and_out_of_the_volume(K, ko, scene, torch_gpu, store):
+    """The 1 cm volume ends at z = 0.78 m: most rays leave the box or cross
+    unobserved blocks, the samples the raycaster steps over without reading
+    the volume.  The maps must still equal the oracle's sample-by-sample march."""
+    p = K.Pipeline(pipeline_config(K, store=store))
+    vol = oracle_volume(ko, kind=STORE_KIND[store])
+    for depth, rgba, pose in frames(scene, 2):
+        w2c = ko.rigid_inverse(pose)
+        ko.integrate(vol, 1, depth, INTR, w2c, ko.integration_options(1000.0), color=rgba)
+        p.integrate(to_dev(torch_gpu, depth), w2c, color=to_dev(torch_gpu, rgba))
+    for level, pose in ((2, scene.pose(2)), (1, scene.pose(1) @ np.diag([1.0, 1.0, 1.0, 1.0]))):
+        s = 1 << level
+        k = tuple(float(np.float32(v) * np.float32(1.0 / s)) for v in INTR)
+        maps = p.raycast(pose, level)
+        pts, nrm, col, samples = ko.raycast(vol, pose, k, W // s, H // s)
+        valid = np.isfinite(pts).all(-1)
+        print(f"level {level}: hit fraction {valid.mean():.3f}, oracle samples per ray {samples / valid.size:.0f}")
+        assert 0.0 < valid.mean() < 0.9
+        assert same_bits(host(maps.points)[..., :3], pts)
+        assert same_bits(host(maps.normals)[..., :3], nrm)
+        assert np.array_equal(host(maps.colors).view(np.uint32), col)
+    p.close()
+
+
+def test_raycast_after_volume_upload(K, ko, torch_gpu):
+    """An uploaded volume rebuilds the occupancy bits the raycaster skips by."""
+    import test_oracle_known_answers as T
+
+    vol, _, _ = T.sphere_volume(ko)
+    cfg = K.PipelineConfig(volume=K.VolumeGeometry((T.SYN_RES,) * 3, T.SYN_VOXEL, T.SYN_ORIGIN, T.SYN_TRUNC),
+                           width=16, height=16, intrinsics=K.CameraIntrinsics(*T.SYN_INTR))
+    p = K.Pipeline(cfg)
+    p.upload_volume(vol.voxels, vol.colors)
+    maps = p.raycast(np.eye(4), 0)
+    pts, nrm, col, _ = ko.raycast(vol, np.eye(4), T.SYN_INTR, 16, 16)
+    assert np.isfinite(pts).all(-1).any()
+    assert same_bits(host(maps.points)[..., :3], pts) and same_bits(host(maps.normals)[..., :3], nrm)
+    p.close()
+
+
 # --- ICP -------------------------------------------------------------------------------
 def tracking_setup(K, ko, scene, torch_gpu, n=2):
     p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL))
