@@ -92,8 +92,11 @@ __global__ void frame_begin_kernel(IcpState* st, int init_mode) {
   }
 }
 
-__global__ void frame_end_kernel(IcpState* st, double* trajectory, long long slot) {
+// The trajectory slot is the device frame counter, so the same launch serves
+// every frame of a captured graph.
+__global__ void frame_end_kernel(IcpState* st, double* trajectory) {
   if (threadIdx.x != 0) return;
+  const long long slot = st->frames_done;
   for (int i = 0; i < 16; ++i) {
     trajectory[slot * 16 + i] = st->pose[i];
     st->prev2_pose[i] = st->prev_pose[i];
@@ -221,6 +224,40 @@ void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing) {
                      c.icp_iterations[l], p, l);
     if (timing) record(ctx, slot++);
   }
+}
+
+void drop_graph(kf_context* ctx) {
+  if (ctx->graph_exec != nullptr) (void)hipGraphExecDestroy(ctx->graph_exec);
+  if (ctx->graph != nullptr) (void)hipGraphDestroy(ctx->graph);
+  ctx->graph_exec = nullptr;
+  ctx->graph = nullptr;
+}
+
+// Reconstruction::initialize / process_frame on the device, enqueued on the
+// context stream (eagerly, or into a stream capture): pyramid, the pose seed,
+// coarse-to-fine tracking (not on the first frame), integration, and the
+// trajectory record.  No call here waits on the device.
+void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color, const kf_icp_options& opt,
+                   bool first) {
+  const bool timing = ctx->cfg.stage_timing != 0;
+  record(ctx, 0);
+  build_pyramid(ctx, depth);
+  record(ctx, 1);
+  const kf_depth_frame frame = frame_from_pyramid(ctx, depth, color);
+  if (first) {
+    // initialize(): integrate the first frame at the identity
+    hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
+                       static_cast<int>(KF_INIT_PREVIOUS));
+    for (int s = 2; s < 2 + 2 * ctx->cfg.levels && timing; ++s) record(ctx, s);
+  } else {
+    hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
+                       ctx->cfg.init_mode == KF_INIT_CONSTANT_VELOCITY ? KF_INIT_CONSTANT_VELOCITY
+                                                                     : KF_INIT_PREVIOUS);
+    track_levels(ctx, to_device_params(opt), timing);
+  }
+  launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
+  hipLaunchKernelGGL(frame_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->trajectory);
+  KF_HIP(hipGetLastError());
 }
 
 void copy_results(const kf_context* ctx, kf_icp_result* results) {
@@ -352,7 +389,14 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     }
     const size_t npx = static_cast<size_t>(c.width) * c.height;
     ctx->pix = dalloc<float4>(npx);
-    ctx->tile_max = dalloc<float>(((c.width + kTile - 1) / kTile) * ((c.height + kTile - 1) / kTile));
+    ctx->raw_depth = dalloc<uint16_t>(npx);
+    ctx->raw_color = dalloc<uint32_t>(npx);
+    {
+      // 16 px tiles, then 4 x 4 and 16 x 16 groups of them, then the frame maximum
+      const int tx = (c.width + kTile - 1) / kTile, ty = (c.height + kTile - 1) / kTile;
+      const int t1x = (tx + 3) / 4, t1y = (ty + 3) / 4, t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
+      ctx->tile_max = dalloc<float>(static_cast<size_t>(tx) * ty + t1x * t1y + t2x * t2y + 1);
+    }
     const size_t nb = static_cast<size_t>((c.resolution[0] + kBrickX - 1) / kBrickX) *
                       ((c.resolution[1] + kBrickY - 1) / kBrickY) *
                       ((c.resolution[2] + kBrickZ - 1) / kBrickZ);
@@ -373,7 +417,7 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->icp = dalloc<IcpState>(1);
     KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
     KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));
-    ctx->traj_capacity = 4096;
+    ctx->traj_capacity = 1 << 15;  // frames before the first regrowth
     ctx->trajectory = dalloc<double>(ctx->traj_capacity * 16);
     for (auto& e : ctx->ev) KF_HIP(hipEventCreate(&e));
     launch_fill_voxels(ctx);
@@ -395,6 +439,9 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
 void kf_destroy(kf_context* ctx) {
   if (ctx == nullptr) return;
   if (ctx->stream != nullptr) (void)hipStreamSynchronize(ctx->stream);
+  drop_graph(ctx);
+  dfree(ctx->raw_depth);
+  dfree(ctx->raw_color);
   dfree(ctx->voxels);
   dfree(ctx->colors);
   for (auto& L : ctx->lv) {
@@ -580,29 +627,38 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
       dfree(ctx->trajectory);
       ctx->trajectory = bigger;
       ctx->traj_capacity *= 2;
+      drop_graph(ctx);  // the captured frame writes the old trajectory buffer
     }
     const bool timing = ctx->cfg.stage_timing != 0;
-    record(ctx, 0);
-    build_pyramid(ctx, depth);
-    record(ctx, 1);
-    const kf_depth_frame frame = frame_from_pyramid(ctx, depth, color);
-    if (ctx->frames == 0) {
-      // initialize(): integrate the first frame at the identity
-      hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
-                         static_cast<int>(KF_INIT_PREVIOUS));
-      for (int s = 2; s < 2 + 2 * ctx->cfg.levels && timing; ++s) record(ctx, s);
-      launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
+    const bool eager = timing || ctx->cfg.count_fused != 0 || ctx->frames == 0;
+    if (eager) {
+      enqueue_frame(ctx, depth, color, *opt, ctx->frames == 0);
     } else {
-      hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
-                         ctx->cfg.init_mode == KF_INIT_CONSTANT_VELOCITY ? KF_INIT_CONSTANT_VELOCITY
-                                                                       : KF_INIT_PREVIOUS);
-      const DeviceIcpParams p = to_device_params(*opt);
-      track_levels(ctx, p, timing);
-      launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
+      // tracking frames replay one captured graph over staging copies of the inputs
+      const size_t npx = static_cast<size_t>(ctx->cfg.width) * ctx->cfg.height;
+      KF_HIP(hipMemcpyAsync(ctx->raw_depth, depth, npx * sizeof(uint16_t), hipMemcpyDeviceToDevice, ctx->stream));
+      if (color != nullptr)
+        KF_HIP(hipMemcpyAsync(ctx->raw_color, color, npx * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+      const bool has_color = color != nullptr;
+      if (ctx->graph_exec == nullptr || has_color != ctx->graph_color ||
+          std::memcmp(&ctx->graph_opt, opt, sizeof(kf_icp_options)) != 0) {
+        drop_graph(ctx);
+        KF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        try {
+          enqueue_frame(ctx, ctx->raw_depth, has_color ? ctx->raw_color : nullptr, *opt, false);
+        } catch (...) {
+          hipGraph_t broken = nullptr;
+          (void)hipStreamEndCapture(ctx->stream, &broken);
+          if (broken != nullptr) (void)hipGraphDestroy(broken);
+          throw;
+        }
+        KF_HIP(hipStreamEndCapture(ctx->stream, &ctx->graph));
+        KF_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
+        ctx->graph_opt = *opt;
+        ctx->graph_color = has_color;
+      }
+      KF_HIP(hipGraphLaunch(ctx->graph_exec, ctx->stream));
     }
-    hipLaunchKernelGGL(frame_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->trajectory,
-                       static_cast<long long>(ctx->frames));
-    KF_HIP(hipGetLastError());
     if (timing) KF_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
     ctx->frames += 1;
     ctx->have_times = timing;

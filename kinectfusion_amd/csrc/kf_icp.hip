@@ -29,7 +29,7 @@ namespace {
 
 constexpr int kSums = 29;
 constexpr int kPartialStride = 32;
-constexpr int kMaxBlocks = 512;
+constexpr int kMaxBlocks = 256;  // partial rows staged in 32 KB of LDS
 
 struct Maps {
   const float4* __restrict__ lv;
@@ -134,40 +134,30 @@ __device__ inline bool last_block(unsigned int* ticket) {
   return is_last;
 }
 
-// Kahan-compensated FP32 sum of the partial rows.  Column c (29 sums and
-// the count) is split over 8 threads: part q adds blocks q, q + 8, ... with
-// Kahan, then thread c adds the 8 part sums (and their compensations) in
-// order q = 0..7.  The order is fixed, so the result does not depend on
-// scheduling; the loads of a part are independent and issue back to back.
+// Kahan-compensated FP32 sum of the partial rows.  The rows are first
+// staged in LDS with coalesced loads.  Column c (29 sums and the count) is
+// split over 8 threads: part q adds rows q, q + 8, ... with Kahan, then
+// thread c adds the 8 part sums (net of their compensations) in order
+// q = 0..7.  The order is fixed, so the result does not depend on scheduling.
 __device__ inline void combine_partials(const float* __restrict__ partial, int nblocks, float* out) {
+  __shared__ float rows[kMaxBlocks * kPartialStride];
   __shared__ float part_sum[8][kPartialStride];
   __shared__ float part_comp[8][kPartialStride];
   __shared__ unsigned part_cnt[8];
+  for (int i = threadIdx.x; i < nblocks * kPartialStride; i += blockDim.x) rows[i] = partial[i];
+  __syncthreads();
   const int col = threadIdx.x & 31, q = threadIdx.x >> 5;  // 256 threads: 32 columns x 8 parts
   float sum = 0.0F, comp = 0.0F;
   unsigned cnt = 0;
   if (col < kSums) {
-    int b = q;
-    for (; b + 24 < nblocks; b += 32) {
-      float v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = partial[(b + 8 * u) * kPartialStride + col];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float y = v[u] - comp;
-        const float t = sum + y;
-        comp = (t - sum) - y;
-        sum = t;
-      }
-    }
-    for (; b < nblocks; b += 8) {
-      const float y = partial[b * kPartialStride + col] - comp;
+    for (int b = q; b < nblocks; b += 8) {
+      const float y = rows[b * kPartialStride + col] - comp;
       const float t = sum + y;
       comp = (t - sum) - y;
       sum = t;
     }
   } else if (col == kSums) {
-    for (int b = q; b < nblocks; b += 8) cnt += __float_as_uint(partial[b * kPartialStride + kSums]);
+    for (int b = q; b < nblocks; b += 8) cnt += __float_as_uint(rows[b * kPartialStride + kSums]);
   }
   part_sum[q][col] = sum;
   part_comp[q][col] = comp;
@@ -176,7 +166,6 @@ __device__ inline void combine_partials(const float* __restrict__ partial, int n
   if (threadIdx.x < kSums) {
     float s2 = 0.0F, c2 = 0.0F;
     for (int k = 0; k < 8; ++k) {
-      // add the part's value net of its own compensation
       const float y = (part_sum[k][threadIdx.x] - part_comp[k][threadIdx.x]) - c2;
       const float t = s2 + y;
       c2 = (t - s2) - y;

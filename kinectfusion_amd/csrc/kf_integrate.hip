@@ -98,6 +98,67 @@ __global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restr
   if (tid == 0) tile_max[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
+// Coarser depth-maximum levels for the cull: 4 x 4 tiles of 16 px (64 px)
+// and 16 x 16 tiles (256 px), plus the frame maximum.  One workgroup.
+__global__ void __launch_bounds__(256) tile_pyramid_kernel(float* __restrict__ tile_max, int tiles_x, int tiles_y) {
+  float* l1 = tile_max + tiles_x * tiles_y;
+  const int t1x = (tiles_x + 3) / 4, t1y = (tiles_y + 3) / 4;
+  float* l2 = l1 + t1x * t1y;
+  const int t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
+  float* all = l2 + t2x * t2y;
+  for (int i = threadIdx.x; i < t1x * t1y; i += 256) {
+    const int x = i % t1x, y = i / t1x;
+    float m = 0.0F;
+    for (int yy = y * 4; yy < min(y * 4 + 4, tiles_y); ++yy)
+      for (int xx = x * 4; xx < min(x * 4 + 4, tiles_x); ++xx) m = fmaxf(m, tile_max[yy * tiles_x + xx]);
+    l1[i] = m;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < t2x * t2y; i += 256) {
+    const int x = i % t2x, y = i / t2x;
+    float m = 0.0F;
+    for (int yy = y * 4; yy < min(y * 4 + 4, t1y); ++yy)
+      for (int xx = x * 4; xx < min(x * 4 + 4, t1x); ++xx) m = fmaxf(m, l1[yy * t1x + xx]);
+    l2[i] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = 0.0F;
+    for (int i = 0; i < t2x * t2y; ++i) m = fmaxf(m, l2[i]);
+    *all = m;
+  }
+}
+
+// Maximum usable depth over the 16 px tiles [tx0, tx1] x [ty0, ty1], read at
+// the coarsest level that still needs at most 4 x 4 lookups (a coarser tile
+// covers more pixels, so the bound stays conservative).
+__device__ inline float region_max(const float* __restrict__ tile_max, int tiles_x, int tiles_y, int tx0, int tx1,
+                                   int ty0, int ty1) {
+  const int t1x = (tiles_x + 3) / 4, t1y = (tiles_y + 3) / 4;
+  const int t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
+  const float* l1 = tile_max + tiles_x * tiles_y;
+  const float* l2 = l1 + t1x * t1y;
+  const float* all = l2 + t2x * t2y;
+  const float* lv = tile_max;
+  int w = tiles_x;
+  for (int level = 0; level < 3; ++level) {
+    if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) <= 16) {
+      float m = 0.0F;
+      for (int y = ty0; y <= ty1; ++y)
+        for (int x = tx0; x <= tx1; ++x) m = fmaxf(m, lv[y * w + x]);
+      return m;
+    }
+    tx0 >>= 2;
+    tx1 >>= 2;
+    ty0 >>= 2;
+    ty1 >>= 2;
+    lv = level == 0 ? l1 : l2;
+    w = level == 0 ? t1x : t2x;
+  }
+  (void)t2y;
+  return *all;
+}
+
 // 2. conservative brick cull.
 __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
                                                           const float* __restrict__ tile_max,
@@ -150,10 +211,9 @@ __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
         ty1 = min(tiles_y - 1, static_cast<int>(floorf((fminf(vmax, 1.0e6F) + 0.5F) / kTile)));
       }
     }
+    if (keep && (tx0 > tx1 || ty0 > ty1)) keep = false;
     if (keep) {
-      float dmax = 0.0F;
-      for (int ty = ty0; ty <= ty1; ++ty)
-        for (int tx = tx0; tx <= tx1; ++tx) dmax = fmaxf(dmax, tile_max[ty * tiles_x + tx]);
+      const float dmax = region_max(tile_max, tiles_x, tiles_y, tx0, tx1, ty0, ty1);
       if (!(dmax > 0.0F)) {
         keep = false;
       } else if (zmin > 1.0e-3F) {
@@ -452,6 +512,7 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
   hipLaunchKernelGGL(pixel_prep_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream,
                      fr.depth, reinterpret_cast<const float4*>(fr.normals), ctx->pix, ctx->tile_max,
                      f, ctx->grid.trunc);
+  hipLaunchKernelGGL(tile_pyramid_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->tile_max, tiles_x, tiles_y);
   const Grid& g = ctx->grid;
   const int nb = ((g.nx + kBrickX - 1) / kBrickX) * ((g.ny + kBrickY - 1) / kBrickY) *
                  ((g.nz + kBrickZ - 1) / kBrickZ);

@@ -271,7 +271,13 @@ struct kf_context {
   float4* colors = nullptr;
   int levels_built = 0;
   kf::LevelBuffers lv[kf::kMaxLevels];
-  uint16_t* raw_depth = nullptr;  // raw depth copy for integration
+  uint16_t* raw_depth = nullptr;  // staging copy of the frame's raw depth (graph input)
+  uint32_t* raw_color = nullptr;  // staging copy of the frame's colour (graph input)
+  // captured per-frame work (tracking frames); rebuilt when its key changes
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  kf_icp_options graph_opt{};
+  bool graph_color = false;
   // integration scratch
   float4* pix = nullptr;          // per pixel: depth m, lambda, weight, valid
   float* tile_max = nullptr;      // per 16x16 tile max usable depth

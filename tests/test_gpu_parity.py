@@ -416,3 +416,27 @@ def test_small_workload_runs_and_is_deterministic(K, scene, torch_gpu):
         outs.append((p.trajectory(), p.host_view()[0]))
         p.close()
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_captured_frames_match_eager_frames(K, scene, torch_gpu):
+    """Tracking frames replay one captured HIP graph; with stage timing on,
+    every frame is enqueued eagerly instead.  Both give byte-identical poses,
+    voxels and colours, also across an ICP-option change (which re-captures)
+    and a frame without colour."""
+    def run(stage_timing):
+        p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL, stage_timing=stage_timing))
+        loose = K.ProjectiveIcpOptions(max_point_distance=0.2)
+        for i in range(6):
+            depth, rgba, _ = scene.frame(i)
+            opts = loose if i == 3 else None
+            color = None if i == 4 else to_dev(torch_gpu, rgba)
+            p.process_frame(to_dev(torch_gpu, depth), color, icp=opts, report=False)
+        p.synchronize()
+        vox, col = p.host_view()
+        out = (p.trajectory(), vox.copy(), col.copy())
+        p.close()
+        return out
+
+    eager, graph = run(True), run(False)
+    assert np.array_equal(eager[0], graph[0])
+    assert np.array_equal(eager[1], graph[1]) and np.array_equal(eager[2], graph[2])
