@@ -107,9 +107,16 @@ typedef struct kf_config {
   int64_t block_capacity;
   /* kf_process_frame */
   int32_t init_mode;
-  int32_t icp_iterations[4]; /* per level; reference default 10/5/4 */
+  int32_t icp_iterations[4]; /* per level; reference default 10/5/4; 0 skips a level */
   int32_t stage_timing;      /* record HIP events per stage in kf_process_frame */
   int32_t count_fused;       /* kf_process_frame accumulates fused-voxel counts */
+  /* KF_INIT_LEARNED (no reference counterpart): per-level iterations run from
+   * the network's proposal (0 skips the level and its raycast), and the
+   * largest final point-to-plane RMS (m) at which the proposal's result is
+   * kept; otherwise the frame is tracked again from the constant-velocity
+   * seed with icp_iterations. */
+  int32_t learned_iterations[4];
+  float learned_max_rms;
 } kf_config;
 
 /* ProjectiveIcpOptions (icp_optimizer.hpp:82-102). */
@@ -185,6 +192,7 @@ typedef struct kf_stage_times {
   float integrate_ms;        /* prep + cull + integrate kernels */
   float integrate_kernel_ms; /* the voxel-update kernel alone */
   float total_ms;
+  float proposal_ms;         /* pose network + seed (KF_INIT_LEARNED) */
 } kf_stage_times;
 
 /* Result of kf_process_frame (Reconstruction::process_frame,
@@ -196,10 +204,27 @@ typedef struct kf_frame_result {
   int32_t pad;
   double pose[16];
   kf_icp_result level[4];
+  /* KF_INIT_LEARNED: the proposal's distance from the final pose and the
+   * final RMS of the tracking run from it */
+  double proposal_translation_error; /* m */
+  double proposal_rotation_error;    /* rad */
+  double learned_rms;                /* m */
 } kf_frame_result;
 
 typedef struct kf_context kf_context;
 typedef struct kf_mesh kf_mesh;
+typedef struct kf_posenet kf_posenet;
+
+/* Pose-proposal network of the learned warm start (no reference
+ * counterpart): training and inference options. */
+typedef struct kf_posenet_config {
+  int32_t batch_capacity; /* largest batch of pairs per call */
+  int32_t pad;
+  uint64_t seed;          /* weight initialisation */
+  float learning_rate;    /* Adam */
+  float beta1, beta2, epsilon;
+  float depth_scale;      /* raw depth units per metre */
+} kf_posenet_config;
 
 /* ---- lifecycle: Pipeline::create (pipeline.hpp:118-121, pipeline.cpp:129-147).
  * No silent host fallback: KF_ERR_UNSUPPORTED when no gfx950 device is
@@ -289,6 +314,22 @@ kf_status kf_volume_upload(kf_context* ctx, const void* voxels, const float* col
 /* Device pointers of the dense volume (float4 colours may be NULL). */
 kf_status kf_volume_device(kf_context* ctx, void** voxels, float** colors);
 
+/* ---- hashed volume (cfg.storage == KF_STORAGE_HASHED): BlockRep
+ * (block_rep.hpp:183-307) with a hash of cfg.hash_buckets buckets over a
+ * pool of cfg.block_capacity 8^3-voxel blocks.  Integration marks the
+ * truncation band of every pixel at half-block steps
+ * (TruncationBandWalk, tsdf_integration.hpp:378-428), allocates new blocks,
+ * and updates only the blocks visited in the frame.  kf_volume_download
+ * returns the dense expansion (unallocated voxels fresh, as
+ * BlockRep::host_dense_view); kf_volume_upload is not available.
+ * stats[0] blocks allocated, [1] requests dropped because the pool or the
+ * hash was full (BlockRep::overflowed), [2] hash entries, [3] blocks
+ * integrated by the last frame. */
+kf_status kf_hashed_stats(kf_context* ctx, int64_t* stats);
+/* Keys (flat x-major block indices, grid.hpp:44-59) of the allocated blocks
+ * in increasing order; *count receives their number. */
+kf_status kf_hashed_blocks(kf_context* ctx, uint32_t* keys, int64_t capacity, int64_t* count);
+
 /* ---- MarchingCubes::extract (marching_cubes.cpp:478-507) on the device:
  * cell classification, stream-compacted triangles and the reference's
  * first-encounter vertex welding order. */
@@ -297,6 +338,46 @@ kf_status kf_mesh_size(const kf_mesh* mesh, int64_t* vertices, int64_t* triangle
 kf_status kf_mesh_copy(const kf_mesh* mesh, float* positions, float* normals, float* colors,
                        uint32_t* triangles);
 void kf_mesh_free(kf_mesh* mesh);
+
+/* ---- learned warm start.  No counterpart in the reference: the network
+ * replaces the tracker's initial pose (Reconstruction::track_pose,
+ * src/app/reconstruction.cpp:218-250), and its fallback reads the final
+ * IcpNormalEquations::mean_squared_residual (icp_correspondence.hpp:133-139).
+ * Input: pairs of 160 x 120 raw depth images (previous, current), the
+ * coarsest level of the 640 x 480 pyramid, fed as metres minus 1.5 (0 where
+ * invalid); output: the twist (omega rad, t m) with T_current =
+ * T_previous [R(omega) | t], the network's six outputs scaled by
+ * (0.1, 0.1, 0.1, 0.03, 0.03, 0.03).  Parameters are packed
+ * per layer as W[out][in] then b[out], convolution inputs ordered
+ * (ky * 3 + kx) * Cin + ci, the flattened 4 x 5 x 128 map in (y, x, c) order.
+ * Buffers passed to these calls are device pointers unless noted; calls on
+ * one network must not overlap. */
+void kf_posenet_config_defaults(kf_posenet_config* cfg);
+int64_t kf_posenet_parameter_count(void);
+kf_status kf_posenet_create(const kf_posenet_config* cfg, kf_posenet** out);
+void kf_posenet_destroy(kf_posenet* net);
+const char* kf_posenet_last_error(const kf_posenet* net);
+/* FP32 master parameters to / from host memory; setting them clears the Adam
+ * moments. */
+kf_status kf_posenet_get_parameters(kf_posenet* net, float* params);
+kf_status kf_posenet_set_parameters(kf_posenet* net, const float* params);
+/* pairs: [batch][2][120][160] u16; twists: [batch][6] floats.  Synchronous. */
+kf_status kf_posenet_forward(kf_posenet* net, const uint16_t* pairs, int32_t batch, float* twists);
+/* Mean squared error of the normalised output against targets [batch][6]
+ * (twists): loss (host) and the gradient of every parameter (host, may be
+ * NULL), without updating.  Synchronous. */
+kf_status kf_posenet_gradients(kf_posenet* net, const uint16_t* pairs, const float* targets, int32_t batch,
+                               float* loss, float* grads);
+/* One Adam step on the batch.  loss (host) may be NULL: then nothing waits. */
+kf_status kf_posenet_train_step(kf_posenet* net, const uint16_t* pairs, const float* targets, int32_t batch,
+                                float* loss);
+/* Adam step size of the following kf_posenet_train_step calls (schedules). */
+kf_status kf_posenet_set_learning_rate(kf_posenet* net, float learning_rate);
+kf_status kf_posenet_synchronize(kf_posenet* net);
+/* Use `net` for KF_INIT_LEARNED frames of `ctx` (NULL detaches).  The
+ * pipeline must have at least 3 levels with a 160 x 120 coarsest level.
+ * The network must outlive its use by the context. */
+kf_status kf_attach_posenet(kf_context* ctx, kf_posenet* net);
 
 #ifdef __cplusplus
 }

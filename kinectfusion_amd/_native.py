@@ -41,7 +41,7 @@ class KfConfig(C.Structure):
         ("cy", C.c_float),
         ("hash_buckets", C.c_int64), ("block_capacity", C.c_int64),
         ("init_mode", C.c_int32), ("icp_iterations", C.c_int32 * 4), ("stage_timing", C.c_int32),
-        ("count_fused", C.c_int32),
+        ("count_fused", C.c_int32), ("learned_iterations", C.c_int32 * 4), ("learned_max_rms", C.c_float),
     ]
 
 
@@ -85,12 +85,21 @@ class KfDepthFrame(C.Structure):
 
 class KfStageTimes(C.Structure):
     _fields_ = [("pyramid_ms", C.c_float), ("raycast_ms", C.c_float), ("icp_ms", C.c_float),
-                ("integrate_ms", C.c_float), ("integrate_kernel_ms", C.c_float), ("total_ms", C.c_float)]
+                ("integrate_ms", C.c_float), ("integrate_kernel_ms", C.c_float), ("total_ms", C.c_float),
+                ("proposal_ms", C.c_float)]
 
 
 class KfFrameResult(C.Structure):
     _fields_ = [("tracked", C.c_int32), ("failed_level", C.c_int32), ("used_learned", C.c_int32),
-                ("pad", C.c_int32), ("pose", C.c_double * 16), ("level", KfIcpResult * 4)]
+                ("pad", C.c_int32), ("pose", C.c_double * 16), ("level", KfIcpResult * 4),
+                ("proposal_translation_error", C.c_double), ("proposal_rotation_error", C.c_double),
+                ("learned_rms", C.c_double)]
+
+
+class KfPosenetConfig(C.Structure):
+    _fields_ = [("batch_capacity", C.c_int32), ("pad", C.c_int32), ("seed", C.c_uint64),
+                ("learning_rate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("depth_scale", C.c_float)]
 
 
 # (name, restype, argtypes) of every exported symbol of include/kf_hip.h
@@ -125,6 +134,21 @@ HIP_SYMBOLS = [
     ("kf_mesh_size", _i32, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     ("kf_mesh_copy", _i32, [_vp, _vp, _vp, _vp, _vp]),
     ("kf_mesh_free", None, [_vp]),
+    ("kf_hashed_stats", _i32, [_vp, C.POINTER(_i64)]),
+    ("kf_hashed_blocks", _i32, [_vp, _vp, _i64, C.POINTER(_i64)]),
+    ("kf_posenet_config_defaults", None, [C.POINTER(KfPosenetConfig)]),
+    ("kf_posenet_parameter_count", _i64, []),
+    ("kf_posenet_create", _i32, [C.POINTER(KfPosenetConfig), C.POINTER(_vp)]),
+    ("kf_posenet_destroy", None, [_vp]),
+    ("kf_posenet_last_error", C.c_char_p, [_vp]),
+    ("kf_posenet_get_parameters", _i32, [_vp, _vp]),
+    ("kf_posenet_set_parameters", _i32, [_vp, _vp]),
+    ("kf_posenet_forward", _i32, [_vp, _vp, _i32, _vp]),
+    ("kf_posenet_gradients", _i32, [_vp, _vp, _vp, _i32, C.POINTER(_f32), _vp]),
+    ("kf_posenet_train_step", _i32, [_vp, _vp, _vp, _i32, C.POINTER(_f32)]),
+    ("kf_posenet_set_learning_rate", _i32, [_vp, _f32]),
+    ("kf_posenet_synchronize", _i32, [_vp]),
+    ("kf_attach_posenet", _i32, [_vp, _vp]),
 ]
 
 SCENE_SYMBOLS = [
@@ -133,6 +157,8 @@ SCENE_SYMBOLS = [
     ("kf_scene_destroy", None, [_vp]),
     ("kf_scene_pose", None, [_vp, C.c_int, _dp]),
     ("kf_scene_render", None, [_vp, _dp, C.c_uint64, C.c_int, _vp, _vp, _vp]),
+    ("kf_scene_description", _vp, [_vp, C.POINTER(C.c_size_t)]),
+    ("kf_scene_render_device", C.c_int, [_vp, _dp, _vp, C.c_int, C.c_int, _vp, _vp, _vp]),
 ]
 
 _hip: C.CDLL | None = None

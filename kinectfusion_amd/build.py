@@ -2,7 +2,8 @@
 
 Produces, in-tree:
   kinectfusion_amd/libkf_hip.so    -- gfx950 HIP kernels + the C ABI (include/kf_hip.h)
-  kinectfusion_amd/libkf_scene.so  -- seeded synthetic RGB-D scenes (host C++)
+  kinectfusion_amd/libkf_scene.so  -- seeded synthetic RGB-D scenes (host C++ renderer and the same
+                                      per-pixel renderer as a gfx950 kernel)
 
 HIP flags: gfx950 only, -O3, no FMA contraction (bit-exact parity with the
 reference's float expression order), IEEE fp32 divide/sqrt (hip default),
@@ -68,7 +69,10 @@ def build(jobs: int = 8) -> dict[str, Path]:
     os.replace(tmp, lib)
     scene = PKG / "libkf_scene.so"
     tmp = PKG / "libkf_scene.so.tmp"
-    _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", "-fopenmp", str(CSRC / "scene.cpp"), "-o", str(tmp)])
+    host_obj = BUILD / "scene.o"
+    _run(["g++", "-std=c++20", "-O2", "-fPIC", "-fopenmp", "-c", str(CSRC / "scene.cpp"), "-o", str(host_obj)])
+    dev_obj = _hip_object("scene_device")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(host_obj), str(dev_obj), "-lgomp", "-o", str(tmp)])
     os.replace(tmp, scene)
     return {"hip": lib, "scene": scene}
 

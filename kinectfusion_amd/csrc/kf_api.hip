@@ -11,6 +11,7 @@
 //    every level succeeded; a failing level leaves its initial pose.
 #include <algorithm>
 #include <cstring>
+#include <vector>
 #include <new>
 #include <string>
 
@@ -122,6 +123,93 @@ __global__ void set_identity_kernel(IcpState* st) {
   }
 }
 
+// FP64 rotation of the twist omega (Rodrigues), row-major 3x3.
+__device__ void rodrigues64(const double w[3], double R[9]) {
+  const double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+  double a, b;
+  if (th < 1e-12) {
+    a = 1.0;
+    b = 0.5;
+  } else {
+    a = sin(th) / th;
+    b = (1.0 - cos(th)) / (th * th);
+  }
+  const double K[9] = {0.0, -w[2], w[1], w[2], 0.0, -w[0], -w[1], w[0], 0.0};
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double k2 = 0.0;
+      for (int q = 0; q < 3; ++q) k2 += K[r * 3 + q] * K[q * 3 + c];
+      R[r * 3 + c] = (r == c ? 1.0 : 0.0) + a * K[r * 3 + c] + b * k2;
+    }
+}
+
+// The learned seed: keep the constant-velocity seed (already in pose) for
+// the fallback, then pose = prev_pose [R(omega) | t].
+__global__ void learned_seed_kernel(IcpState* st, const float* twist) {
+  if (threadIdx.x != 0) return;
+  const double w[3] = {twist[0], twist[1], twist[2]};
+  double R[9];
+  rodrigues64(w, R);
+  double D[16] = {R[0], R[1], R[2], twist[3], R[3], R[4], R[5], twist[4], R[6], R[7], R[8], twist[5],
+                  0.0,  0.0,  0.0,  1.0};
+  for (int i = 0; i < 16; ++i) st->seed_pose[i] = st->pose[i];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      double s = 0.0;
+      for (int q = 0; q < 4; ++q) s += st->prev_pose[r * 4 + q] * D[q * 4 + c];
+      st->proposal[r * 4 + c] = s;
+      st->pose[r * 4 + c] = s;
+    }
+  for (int l = 0; l < kMaxLevels; ++l) {
+    st->results[l] = kf_icp_result{};
+    st->results[l].status = -1;
+  }
+  st->used_learned = 0;
+  st->learned_rms = -1.0;
+}
+
+// After the run from the proposal: keep it when no level failed and the
+// final point-to-plane RMS is at most max_rms; otherwise restore the
+// constant-velocity seed for the full run.  Kept frames mark the frame
+// failed so the full run's kernels return at once (learned_end_kernel
+// clears the mark).
+__global__ void learned_check_kernel(IcpState* st, int finest, float max_rms) {
+  if (threadIdx.x != 0) return;
+  const kf_icp_result& r = st->results[finest];
+  const double rms = r.mean_squared_residual >= 0.0 ? sqrt(r.mean_squared_residual) : 1e300;
+  st->learned_rms = rms;
+  if (st->frame_failed == 0 && (r.status == 0 || r.status == 1) && rms <= static_cast<double>(max_rms)) {
+    st->used_learned = 1;
+    st->frame_failed = 1;
+  } else {
+    st->used_learned = 0;
+    st->frame_failed = 0;
+    st->failed_level = -1;
+    for (int i = 0; i < 16; ++i) st->pose[i] = st->seed_pose[i];
+  }
+}
+
+__global__ void learned_end_kernel(IcpState* st) {
+  if (threadIdx.x != 0) return;
+  if (st->used_learned != 0) {
+    st->frame_failed = 0;
+    st->failed_level = -1;
+  }
+  // the proposal's distance from the final pose: E = proposal^-1 pose
+  double inv[16];
+  rigid_inverse64(st->proposal, inv);
+  double E[16];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      double s = 0.0;
+      for (int q = 0; q < 4; ++q) s += inv[r * 4 + q] * st->pose[q * 4 + c];
+      E[r * 4 + c] = s;
+    }
+  st->proposal_err_t = sqrt(E[3] * E[3] + E[7] * E[7] + E[11] * E[11]);
+  const double ct = fmin(1.0, fmax(-1.0, 0.5 * (E[0] + E[5] + E[10] - 1.0)));
+  st->proposal_err_r = acos(ct);
+}
+
 template <int Kind>
 __global__ void count_observed_kernel(const typename VoxelIO<Kind>::Raw* v, size_t n,
                                       unsigned long long* out) {
@@ -153,7 +241,16 @@ void validate(const kf_config& c) {
           "Volume origin must be finite");
   require(c.voxel_store >= 0 && c.voxel_store <= 2, "Unknown voxel store");
   require(c.color_store == KF_COLOR_FLOAT || c.color_store == KF_COLOR_NONE, "Unknown color store");
-  require(c.storage == KF_STORAGE_DENSE, "Only the dense storage layout is built into this context");
+  require(c.storage == KF_STORAGE_DENSE || c.storage == KF_STORAGE_HASHED, "Unknown storage layout");
+  if (c.storage == KF_STORAGE_HASHED) {
+    // the hash holds every allocated block plus at most one frame's new
+    // requests once the pool is full, at a load of at most 7/8
+    const int64_t b = c.hash_buckets;
+    require(b >= 1024 && b <= (int64_t{1} << 30) && (b & (b - 1)) == 0,
+            "Hash buckets must be a power of two in [2^10, 2^30]");
+    require(c.block_capacity >= 1 && c.block_capacity <= (int64_t{1} << 26), "Block capacity must be in [1, 2^26]");
+    require(2 * c.block_capacity <= b, "Hash buckets must be at least twice the block capacity");
+  }
   require(c.tsdf_rule == KF_RULE_CLASSIC || c.tsdf_rule == KF_RULE_ANGLE_WEIGHTED, "Unknown TSDF rule");
   // TsdfIntegrator::validated (tsdf_integration.cpp) and depth options
   require(c.depth_scale > 0.0F, "Depth scale must be positive");
@@ -173,7 +270,24 @@ void validate(const kf_config& c) {
   require(c.width > 0 && c.height > 0 && c.width <= 8192 && c.height <= 8192, "Camera size is invalid");
   require(c.fx > 0.0F && c.fy > 0.0F, "Focal lengths must be positive");
   require(c.init_mode >= KF_INIT_PREVIOUS && c.init_mode <= KF_INIT_LEARNED, "Unknown init mode");
-  for (int l = 0; l < c.levels; ++l) require(c.icp_iterations[l] >= 1, "ICP iterations must be >= 1");
+  {
+    // 0 iterations skip a level (and its raycast); at least one level runs
+    int running = 0;
+    for (int l = 0; l < c.levels; ++l) {
+      require(c.icp_iterations[l] >= 0, "ICP iterations must be >= 0");
+      running += c.icp_iterations[l] > 0 ? 1 : 0;
+    }
+    require(running > 0, "ICP must run at least one level");
+  }
+  if (c.init_mode == KF_INIT_LEARNED) {
+    int any = 0;
+    for (int l = 0; l < c.levels; ++l) {
+      require(c.learned_iterations[l] >= 0, "Learned-seed iterations must be >= 0");
+      any += c.learned_iterations[l] > 0 ? 1 : 0;
+    }
+    require(any > 0, "Learned-seed iterations must run at least one level");
+    require(c.learned_max_rms > 0.0F, "Learned-seed RMS threshold must be positive");
+  }
 }
 
 void upload_pose(kf_context* ctx, const double* pose) {
@@ -212,18 +326,28 @@ void record(kf_context* ctx, int slot) {
 }
 
 // Coarse-to-fine tracking from the device pose (track_pose).
-void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing) {
+// Coarse-to-fine tracking with per-level iteration counts (0 skips a level
+// and its raycast).
+void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing, const int32_t* iterations) {
   const kf_config& c = ctx->cfg;
   int slot = 2;
   for (int l = c.levels - 1; l >= 0; --l) {
     LevelBuffers& L = ctx->lv[l];
-    launch_icp_begin_level(ctx, l);
-    launch_raycast(ctx, l, nullptr, nullptr, true);
+    if (iterations[l] > 0) {
+      launch_icp_begin_level(ctx, l);
+      launch_raycast(ctx, l, nullptr, nullptr, true);
+    }
     if (timing) record(ctx, slot++);
-    launch_icp_level(ctx, L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k,
-                     c.icp_iterations[l], p, l);
+    if (iterations[l] > 0) {
+      launch_icp_level(ctx, L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k, iterations[l], p,
+                       l);
+    }
     if (timing) record(ctx, slot++);
   }
+}
+
+bool learned_active(const kf_context* ctx) {
+  return ctx->cfg.init_mode == KF_INIT_LEARNED && ctx->posenet != nullptr;
 }
 
 void drop_graph(kf_context* ctx) {
@@ -249,15 +373,39 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
                        static_cast<int>(KF_INIT_PREVIOUS));
     for (int s = 2; s < 2 + 2 * ctx->cfg.levels && timing; ++s) record(ctx, s);
+  } else if (learned_active(ctx)) {
+    // learned warm start: propose, track from the proposal, fall back to the
+    // constant-velocity seed with the full schedule when it does not hold
+    const kf_config& c = ctx->cfg;
+    hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
+                       static_cast<int>(KF_INIT_CONSTANT_VELOCITY));
+    posenet_propose(ctx->posenet, ctx->stream, ctx->prev_coarse, ctx->lv[c.levels - 1].depth, ctx->twist);
+    hipLaunchKernelGGL(learned_seed_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->twist);
+    record(ctx, 14);
+    const DeviceIcpParams p = to_device_params(opt);
+    track_levels(ctx, p, timing, c.learned_iterations);
+    int finest = 0;
+    while (c.learned_iterations[finest] == 0) ++finest;
+    hipLaunchKernelGGL(learned_check_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, finest,
+                       c.learned_max_rms);
+    track_levels(ctx, p, false, c.icp_iterations);
+    hipLaunchKernelGGL(learned_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
+    record(ctx, 2 * c.levels + 1);  // the fallback run counts as tracking time
   } else {
     hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
                        ctx->cfg.init_mode == KF_INIT_CONSTANT_VELOCITY ? KF_INIT_CONSTANT_VELOCITY
                                                                      : KF_INIT_PREVIOUS);
-    track_levels(ctx, to_device_params(opt), timing);
+    track_levels(ctx, to_device_params(opt), timing, ctx->cfg.icp_iterations);
   }
+  ctx->timed_learned = timing && !first && learned_active(ctx);
   launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
   hipLaunchKernelGGL(frame_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->trajectory);
   KF_HIP(hipGetLastError());
+  if (ctx->cfg.init_mode == KF_INIT_LEARNED && ctx->prev_coarse != nullptr) {
+    const LevelBuffers& L = ctx->lv[ctx->cfg.levels - 1];
+    KF_HIP(hipMemcpyAsync(ctx->prev_coarse, L.depth, sizeof(uint16_t) * L.w * L.h, hipMemcpyDeviceToDevice,
+                          ctx->stream));
+  }
 }
 
 void copy_results(const kf_context* ctx, kf_icp_result* results) {
@@ -318,6 +466,11 @@ void kf_config_defaults(kf_config* c) {
   c->icp_iterations[3] = 4;
   c->stage_timing = 0;
   c->count_fused = 0;
+  c->learned_iterations[0] = 4;
+  c->learned_iterations[1] = 0;
+  c->learned_iterations[2] = 0;
+  c->learned_iterations[3] = 0;
+  c->learned_max_rms = 0.01F;
 }
 
 void kf_icp_options_defaults(kf_icp_options* o) {
@@ -366,7 +519,20 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->grid = Grid{static_cast<int>(c.resolution[0]), static_cast<int>(c.resolution[1]),
                      static_cast<int>(c.resolution[2]), c.voxel_size, c.origin[0], c.origin[1],
                      c.origin[2], c.truncation_distance};
-    ctx->n_voxels = static_cast<size_t>(c.resolution[0]) * c.resolution[1] * c.resolution[2];
+    if (c.storage == KF_STORAGE_HASHED) {
+      // BlockRep (block_rep.hpp:183-307): a pool of 8^3-voxel blocks and the
+      // hash from block keys to pool slots
+      ctx->buckets = static_cast<size_t>(c.hash_buckets);
+      ctx->capacity = static_cast<size_t>(c.block_capacity);
+      ctx->n_voxels = ctx->capacity * kBlockVoxels;
+      ctx->hash_table = dalloc<unsigned long long>(ctx->buckets);
+      ctx->hash_stamp = dalloc<uint32_t>(ctx->buckets);
+      ctx->block_of_slot = dalloc<uint32_t>(ctx->capacity);
+      ctx->work = dalloc<uint32_t>(ctx->capacity);
+      ctx->sparse = dalloc<uint32_t>(8);
+    } else {
+      ctx->n_voxels = static_cast<size_t>(c.resolution[0]) * c.resolution[1] * c.resolution[2];
+    }
     ctx->voxels = dalloc<uint8_t>(ctx->n_voxels * voxel_bytes(c.voxel_store));
     if (c.color_store == KF_COLOR_FLOAT) ctx->colors = dalloc<float4>(ctx->n_voxels);
     int w = c.width, h = c.height;
@@ -397,10 +563,12 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
       const int t1x = (tx + 3) / 4, t1y = (ty + 3) / 4, t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
       ctx->tile_max = dalloc<float>(static_cast<size_t>(tx) * ty + t1x * t1y + t2x * t2y + 1);
     }
-    const size_t nb = static_cast<size_t>((c.resolution[0] + kBrickX - 1) / kBrickX) *
-                      ((c.resolution[1] + kBrickY - 1) / kBrickY) *
-                      ((c.resolution[2] + kBrickZ - 1) / kBrickZ);
-    ctx->bricks = dalloc<uint32_t>(nb);
+    if (c.storage == KF_STORAGE_DENSE) {
+      const size_t nb = static_cast<size_t>((c.resolution[0] + kBrickX - 1) / kBrickX) *
+                        ((c.resolution[1] + kBrickY - 1) / kBrickY) *
+                        ((c.resolution[2] + kBrickZ - 1) / kBrickZ);
+      ctx->bricks = dalloc<uint32_t>(nb);
+    }
     {
       const BlockMap bm = block_map(ctx->grid);
       ctx->block_words = (static_cast<size_t>(bm.bx) * bm.by * bm.bz + 31) / 32;
@@ -417,10 +585,17 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->icp = dalloc<IcpState>(1);
     KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
     KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));
+    if (c.init_mode == KF_INIT_LEARNED) {
+      const LevelBuffers& L = ctx->lv[c.levels - 1];
+      ctx->prev_coarse = dalloc<uint16_t>(static_cast<size_t>(L.w) * L.h);
+      KF_HIP(hipMemsetAsync(ctx->prev_coarse, 0, sizeof(uint16_t) * L.w * L.h, ctx->stream));
+      ctx->twist = dalloc<float>(8);
+    }
     ctx->traj_capacity = 1 << 15;  // frames before the first regrowth
     ctx->trajectory = dalloc<double>(ctx->traj_capacity * 16);
     for (auto& e : ctx->ev) KF_HIP(hipEventCreate(&e));
     launch_fill_voxels(ctx);
+    if (hashed(ctx)) launch_fill_pool(ctx);
     launch_skip_map(ctx);
     hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     KF_HIP(hipStreamSynchronize(ctx->stream));
@@ -463,6 +638,13 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->ticket);
   dfree(ctx->icp);
   dfree(ctx->trajectory);
+  dfree(ctx->prev_coarse);
+  dfree(ctx->twist);
+  dfree(ctx->hash_table);
+  dfree(ctx->hash_stamp);
+  dfree(ctx->block_of_slot);
+  dfree(ctx->work);
+  dfree(ctx->sparse);
   if (ctx->icp_host != nullptr) (void)hipHostFree(ctx->icp_host);
   for (auto& e : ctx->ev)
     if (e != nullptr) (void)hipEventDestroy(e);
@@ -481,6 +663,7 @@ kf_status kf_synchronize(kf_context* ctx) {
 kf_status kf_reset(kf_context* ctx) {
   return guard(ctx, [&] {
     launch_fill_voxels(ctx);
+    if (hashed(ctx)) launch_fill_pool(ctx);
     KF_HIP(hipMemsetAsync(ctx->obs_bits, 0, ctx->block_words * 4, ctx->stream));
     launch_skip_map(ctx);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
@@ -605,7 +788,7 @@ kf_status kf_track(kf_context* ctx, const kf_icp_options* opt, double* pose_inou
     const DeviceIcpParams p = to_device_params(*opt);
     upload_pose(ctx, pose_inout);
     KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
-    track_levels(ctx, p, false);
+    track_levels(ctx, p, false, ctx->cfg.icp_iterations);
     fetch_state(ctx);
     std::memcpy(pose_inout, ctx->icp_host->pose, 16 * sizeof(double));
     if (results != nullptr) copy_results(ctx, results);
@@ -669,6 +852,12 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
       result->tracked = ctx->icp_host->frame_failed ? 0 : 1;
       std::memcpy(result->pose, ctx->icp_host->pose, sizeof(result->pose));
       if (ctx->frames > 1) copy_results(ctx, result->level);
+      if (ctx->frames > 1 && learned_active(ctx)) {
+        result->used_learned = ctx->icp_host->used_learned;
+        result->proposal_translation_error = ctx->icp_host->proposal_err_t;
+        result->proposal_rotation_error = ctx->icp_host->proposal_err_r;
+        result->learned_rms = ctx->icp_host->learned_rms;
+      }
     }
   });
 }
@@ -698,6 +887,10 @@ kf_status kf_last_stage_times(kf_context* ctx, kf_stage_times* out) {
     // events 2.. alternate: after raycast, after icp (per level); the first
     // raycast starts at event 1
     int prev = 1;
+    if (ctx->timed_learned) {
+      t.proposal_ms = ms(1, 14);
+      prev = 14;
+    }
     for (int l = 0; l < ctx->cfg.levels; ++l) {
       t.raycast_ms += ms(prev, 2 + 2 * l);
       t.icp_ms += ms(2 + 2 * l, 3 + 2 * l);
@@ -749,13 +942,37 @@ kf_status kf_observed_voxel_count(kf_context* ctx, int64_t* count) {
 
 kf_status kf_volume_bytes(const kf_context* ctx, size_t* voxel_bytes_out, size_t* color_bytes) {
   if (ctx == nullptr) return KF_ERR_INVALID_ARGUMENT;
-  if (voxel_bytes_out != nullptr) *voxel_bytes_out = ctx->n_voxels * voxel_bytes(ctx->cfg.voxel_store);
-  if (color_bytes != nullptr) *color_bytes = ctx->colors != nullptr ? ctx->n_voxels * sizeof(float4) : 0;
+  const kf_config& c = ctx->cfg;
+  const size_t dense = static_cast<size_t>(c.resolution[0]) * c.resolution[1] * c.resolution[2];
+  if (voxel_bytes_out != nullptr) *voxel_bytes_out = dense * voxel_bytes(c.voxel_store);
+  if (color_bytes != nullptr) *color_bytes = ctx->colors != nullptr ? dense * sizeof(float4) : 0;
   return KF_OK;
 }
 
 kf_status kf_volume_download(kf_context* ctx, void* voxels, float* colors) {
   return guard(ctx, [&] {
+    if (hashed(ctx)) {
+      // stage the dense expansion on the device, then copy it out
+      size_t vb = 0, cb = 0;
+      kf_volume_bytes(ctx, &vb, &cb);
+      void* dv = dalloc<uint8_t>(vb);
+      float4* dc = cb != 0 ? dalloc<float4>(cb / sizeof(float4)) : nullptr;
+      try {
+        launch_expand_hashed(ctx, dv, dc);
+        if (voxels != nullptr) KF_HIP(hipMemcpyAsync(voxels, dv, vb, hipMemcpyDeviceToHost, ctx->stream));
+        if (colors != nullptr && dc != nullptr)
+          KF_HIP(hipMemcpyAsync(colors, dc, cb, hipMemcpyDeviceToHost, ctx->stream));
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+      } catch (...) {
+        (void)hipStreamSynchronize(ctx->stream);
+        dfree(dv);
+        dfree(dc);
+        throw;
+      }
+      dfree(dv);
+      dfree(dc);
+      return;
+    }
     if (voxels != nullptr)
       KF_HIP(hipMemcpyAsync(voxels, ctx->voxels, ctx->n_voxels * voxel_bytes(ctx->cfg.voxel_store),
                             hipMemcpyDeviceToHost, ctx->stream));
@@ -768,6 +985,7 @@ kf_status kf_volume_download(kf_context* ctx, void* voxels, float* colors) {
 
 kf_status kf_volume_upload(kf_context* ctx, const void* voxels, const float* colors) {
   return guard(ctx, [&] {
+    if (hashed(ctx)) fail(KF_ERR_UNSUPPORTED, "kf_volume_upload: a hashed volume is built by integration only");
     if (voxels != nullptr)
       KF_HIP(hipMemcpyAsync(ctx->voxels, voxels, ctx->n_voxels * voxel_bytes(ctx->cfg.voxel_store),
                             hipMemcpyHostToDevice, ctx->stream));
@@ -787,3 +1005,50 @@ kf_status kf_volume_device(kf_context* ctx, void** voxels, float** colors) {
 }
 
 }  // extern "C"
+
+extern "C" kf_status kf_hashed_stats(kf_context* ctx, int64_t* stats) {
+  return guard(ctx, [&] {
+    require(stats != nullptr, "kf_hashed_stats: null argument");
+    require(hashed(ctx), "kf_hashed_stats: the volume is dense");
+    uint32_t h[8];
+    KF_HIP(hipMemcpyAsync(h, ctx->sparse, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    stats[0] = std::min<int64_t>(h[0], static_cast<int64_t>(ctx->capacity));
+    stats[1] = h[4];
+    stats[2] = h[2];
+    stats[3] = h[1];
+  });
+}
+
+extern "C" kf_status kf_hashed_blocks(kf_context* ctx, uint32_t* keys, int64_t capacity, int64_t* count) {
+  return guard(ctx, [&] {
+    require(hashed(ctx), "kf_hashed_blocks: the volume is dense");
+    int64_t stats[4];
+    const kf_status st = kf_hashed_stats(ctx, stats);
+    if (st != KF_OK) fail(st, ctx->error);
+    const int64_t n = stats[0];
+    if (count != nullptr) *count = n;
+    if (keys == nullptr || n == 0) return;
+    std::vector<uint32_t> host(static_cast<size_t>(n));
+    KF_HIP(hipMemcpyAsync(host.data(), ctx->block_of_slot, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    std::sort(host.begin(), host.end());
+    std::memcpy(keys, host.data(), std::min<int64_t>(n, capacity) * sizeof(uint32_t));
+  });
+}
+
+extern "C" kf_status kf_attach_posenet(kf_context* ctx, kf_posenet* net) {
+  return guard(ctx, [&] {
+    if (net != nullptr) {
+      require(ctx->cfg.init_mode == KF_INIT_LEARNED && ctx->prev_coarse != nullptr,
+              "kf_attach_posenet: the context was not created with KF_INIT_LEARNED");
+      const LevelBuffers& L = ctx->lv[ctx->cfg.levels - 1];
+      require(ctx->cfg.levels >= 3 && posenet_input_matches(L.w, L.h),
+              "kf_attach_posenet: the coarsest pyramid level must be 160 x 120");
+    }
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    ctx->posenet = net;
+  });
+}

@@ -238,6 +238,50 @@ __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
 }
 
 // 3. integrate the listed bricks.
+// observe + fuse of one voxel (tsdf_integration.hpp:184-252); true when fused.
+template <int Kind, bool Color, bool Count>
+__device__ inline bool fuse_voxel(const Grid& g, const FrameParams& f, const Rigid& w2c,
+                                  const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
+                                  typename VoxelIO<Kind>::Raw* __restrict__ vp, float4* __restrict__ cvp, int x,
+                                  int y, int z, unsigned int& my_fused, unsigned int& my_color) {
+  using IO = VoxelIO<Kind>;
+  const V3 cp = xform(w2c, g.center(x, y, z));
+  if (cp.z <= 0.0F) return false;
+  float u, v;
+  project(f.k, cp, u, v);
+  int col, row;
+  if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) return false;
+  const size_t pi = static_cast<size_t>(row) * f.w + col;
+  const float4 p = pix[pi];
+  if (p.x < 0.0F) return false;
+  const float sd = f.projective ? p.x - (norm3(cp) / p.y) : p.x - cp.z;
+  if (sd < -p.w) return false;
+  const float tsdf = fclamp_ref(sd / p.w, -1.0F, 1.0F);
+  const float w = p.z;
+  if (w <= 0.0F) return false;
+  float t0, w0;
+  bool obs, fin;
+  IO::unpack(*vp, t0, w0, obs, fin);
+  const float avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
+  const float tw = fmin_ref(w0 + w, f.max_weight);
+  *vp = IO::pack(avg, tw);
+  if (Count) ++my_fused;
+  if (Color) {
+    if (color_img != nullptr && sd <= p.w * 0.5F) {
+      const uint32_t px = color_img[pi];
+      const V3 c = mk3(static_cast<float>(px & 0xFFU), static_cast<float>((px >> 8U) & 0xFFU),
+                       static_cast<float>((px >> 16U) & 0xFFU));
+      const float4 old = *cvp;
+      const float cw = old.w;
+      const float den = cw + w;
+      *cvp = make_float4(((old.x * cw) + (c.x * w)) / den, ((old.y * cw) + (c.y * w)) / den,
+                         ((old.z * cw) + (c.z * w)) / den, fmin_ref(cw + w, f.max_weight));
+      if (Count) ++my_color;
+    }
+  }
+  return true;
+}
+
 // The voxel updates of one (x, y) column of a brick; true when any fused.
 template <int Kind, bool Color, bool Count>
 __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, const Rigid& w2c,
@@ -245,49 +289,13 @@ __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, con
                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
                                         float4* __restrict__ colors, int x, int y, int bz,
                                         unsigned int& my_fused, unsigned int& my_color) {
-  using IO = VoxelIO<Kind>;
   bool any = false;
-  {
-    for (int zz = 0; zz < kBrickZ; ++zz) {
-      const int z = bz * kBrickZ + zz;
-      if (z >= g.nz) break;
-      const V3 cp = xform(w2c, g.center(x, y, z));
-      if (cp.z <= 0.0F) continue;
-      float u, v;
-      project(f.k, cp, u, v);
-      int col, row;
-      if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) continue;
-      const size_t pi = static_cast<size_t>(row) * f.w + col;
-      const float4 p = pix[pi];
-      if (p.x < 0.0F) continue;
-      const float sd = f.projective ? p.x - (norm3(cp) / p.y) : p.x - cp.z;
-      if (sd < -p.w) continue;
-      const float tsdf = fclamp_ref(sd / p.w, -1.0F, 1.0F);
-      const float w = p.z;
-      if (w <= 0.0F) continue;
-      const size_t vi = g.flat(x, y, z);
-      float t0, w0;
-      bool obs, fin;
-      IO::unpack(vox[vi], t0, w0, obs, fin);
-      const float avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
-      const float tw = fmin_ref(w0 + w, f.max_weight);
-      vox[vi] = IO::pack(avg, tw);
-      any = true;
-      if (Count) ++my_fused;
-      if (Color) {
-        if (color_img != nullptr && sd <= p.w * 0.5F) {
-          const uint32_t px = color_img[pi];
-          const V3 c = mk3(static_cast<float>(px & 0xFFU), static_cast<float>((px >> 8U) & 0xFFU),
-                           static_cast<float>((px >> 16U) & 0xFFU));
-          const float4 old = colors[vi];
-          const float cw = old.w;
-          const float den = cw + w;
-          colors[vi] = make_float4(((old.x * cw) + (c.x * w)) / den, ((old.y * cw) + (c.y * w)) / den,
-                                   ((old.z * cw) + (c.z * w)) / den, fmin_ref(cw + w, f.max_weight));
-          if (Count) ++my_color;
-        }
-      }
-    }
+  for (int zz = 0; zz < kBrickZ; ++zz) {
+    const int z = bz * kBrickZ + zz;
+    if (z >= g.nz) break;
+    const size_t vi = g.flat(x, y, z);
+    any |= fuse_voxel<Kind, Color, Count>(g, f, w2c, pix, color_img, vox + vi, Color ? colors + vi : nullptr, x, y,
+                                          z, my_fused, my_color);
   }
   return any;
 }
@@ -418,6 +426,229 @@ __global__ void fill_voxels_kernel(typename VoxelIO<Kind>::Raw* v, float4* color
   }
 }
 
+// ---- hashed volume: TruncationBandWalk + BlockRep::integrate
+// (tsdf_integration.hpp:378-428, block_rep.hpp:244-307) ----------------------
+__global__ void sparse_begin_kernel(uint32_t* sp) {
+  if (threadIdx.x == 0) {
+    sp[3] += 1U;  // frame id
+    sp[1] = 0U;   // work count
+  }
+}
+
+// RigidTransform inverse {R^T, -R^T t} in float (vector.hpp:214-224).
+__device__ inline Rigid rigid_inverse_f(const Rigid& m) {
+  Rigid r;
+  r.r0 = mk3(m.r0.x, m.r1.x, m.r2.x);
+  r.r1 = mk3(m.r0.y, m.r1.y, m.r2.y);
+  r.r2 = mk3(m.r0.z, m.r1.z, m.r2.z);
+  const V3 t = rot(r, m.t);
+  r.t = mk3(-t.x, -t.y, -t.z);
+  return r;
+}
+
+// Finds or inserts `key`; its bucket, or kNoSlot when the table or the pool
+// is exhausted (the request is counted as overflow).  Entries only go from
+// empty to (key, pending) here, so a stale read can only see "empty", which
+// the compare-and-swap corrects.
+__device__ inline uint32_t hash_insert(const HashView& hv, uint32_t key, uint32_t* sp, uint32_t entry_limit,
+                                       uint32_t capacity) {
+  uint32_t h = hv.bucket(key);
+  for (uint32_t probe = 0; probe <= hv.mask; ++probe) {
+    unsigned long long e = hv.table[h];
+    if (e == kHashEmpty) {
+      if (sp[2] >= entry_limit || sp[0] >= capacity) break;
+      const unsigned long long mine = (static_cast<unsigned long long>(key) << 32) | kPendingSlot;
+      const unsigned long long old = atomicCAS(&hv.table[h], kHashEmpty, mine);
+      if (old == kHashEmpty) {
+        atomicAdd(&sp[2], 1U);
+        return h;
+      }
+      e = old;
+    }
+    if (static_cast<uint32_t>(e >> 32) == key) return h;
+    h = (h + 1) & hv.mask;
+  }
+  atomicAdd(&sp[4], 1U);
+  return kNoSlot;
+}
+
+// Marks the blocks of every pixel's truncation band (visit_pixel): samples
+// from d - band to d + band every `step`, skipping a block equal to the
+// pixel's previous visit.
+__global__ void __launch_bounds__(256) band_walk_kernel(Grid g, FrameParams f, const uint16_t* __restrict__ depth,
+                                                         HashView hv, uint32_t* __restrict__ stamp,
+                                                         uint32_t* __restrict__ sp, uint32_t entry_limit,
+                                                         uint32_t capacity, float step) {
+  Rigid w2c;
+  if (!block_w2c(f, w2c)) return;
+  const Rigid c2w = rigid_inverse_f(w2c);
+  const int col = blockIdx.x * kTile + threadIdx.x, row = blockIdx.y * kTile + threadIdx.y;
+  if (col >= f.w || row >= f.h) return;
+  float measured;
+  if (!depth_in_range(depth[static_cast<size_t>(row) * f.w + col], f.scale, f.lo, f.hi, measured)) return;
+  const float band = f.dist_scaled ? g.trunc + (f.trunc_scale * measured) : g.trunc;
+  const int samples = static_cast<int>((2.0F * band) / step) + 1;
+  const float inv_voxel = 1.0F / g.voxel;
+  const uint32_t frame = sp[3];
+  uint32_t previous = kNoSlot;
+  for (int s = 0; s <= samples; ++s) {
+    const float d = (measured - band) + (static_cast<float>(s) * step);
+    const V3 world = xform(c2w, back_project(f.k, static_cast<float>(col), static_cast<float>(row), d));
+    const V3 gc = scale3(sub3(world, mk3(g.ox, g.oy, g.oz)), inv_voxel);
+    if (!(gc.x >= 0.0F && gc.y >= 0.0F && gc.z >= 0.0F && gc.x < static_cast<float>(g.nx) &&
+          gc.y < static_cast<float>(g.ny) && gc.z < static_cast<float>(g.nz)))
+      continue;
+    const uint32_t key = hv.key_of(static_cast<int>(gc.x), static_cast<int>(gc.y), static_cast<int>(gc.z));
+    if (key == previous) continue;
+    previous = key;
+    const uint32_t b = hash_insert(hv, key, sp, entry_limit, capacity);
+    if (b != kNoSlot) stamp[b] = frame;
+  }
+}
+
+// Pool slots for this frame's new blocks and the frame's work list (each
+// visited block once).  Appends are aggregated per wave.
+__global__ void __launch_bounds__(256) assign_slots_kernel(HashView hv, const uint32_t* __restrict__ stamp,
+                                                            uint32_t* __restrict__ sp,
+                                                            uint32_t* __restrict__ block_of_slot,
+                                                            uint32_t* __restrict__ work, uint32_t capacity,
+                                                            uint32_t buckets) {
+  const uint32_t frame = sp[3];
+  const int lane = threadIdx.x & 63;
+  const uint32_t stride = gridDim.x * 256U;
+  const uint32_t rounds = (buckets + stride - 1) / stride;
+  for (uint32_t r = 0; r < rounds; ++r) {
+    const uint32_t b = r * stride + blockIdx.x * 256U + threadIdx.x;
+    unsigned long long e = kHashEmpty;
+    bool visited = false;
+    if (b < buckets) {
+      e = hv.table[b];
+      visited = e != kHashEmpty && stamp[b] == frame;
+    }
+    uint32_t slot = visited ? static_cast<uint32_t>(e) : kNoSlot;
+    const bool fresh = visited && slot == kPendingSlot;
+    const unsigned long long fm = __ballot(fresh);
+    uint32_t base = 0;
+    if (lane == 0 && fm != 0ULL) base = atomicAdd(&sp[0], static_cast<uint32_t>(__popcll(fm)));
+    base = __shfl(base, 0);
+    if (fresh) {
+      const uint32_t s = base + static_cast<uint32_t>(__popcll(fm & ((1ULL << lane) - 1ULL)));
+      const uint32_t key = static_cast<uint32_t>(e >> 32);
+      if (s < capacity) {
+        slot = s;
+        block_of_slot[s] = key;
+      } else {
+        slot = kNoSlot;
+        atomicAdd(&sp[4], 1U);
+      }
+      hv.table[b] = (static_cast<unsigned long long>(key) << 32) | slot;
+    }
+    const bool listed = visited && slot != kNoSlot;
+    const unsigned long long lm = __ballot(listed);
+    uint32_t wbase = 0;
+    if (lane == 0 && lm != 0ULL) wbase = atomicAdd(&sp[1], static_cast<uint32_t>(__popcll(lm)));
+    wbase = __shfl(wbase, 0);
+    if (listed) work[wbase + __popcll(lm & ((1ULL << lane) - 1ULL))] = slot;
+  }
+}
+
+// Every voxel of the listed blocks (BlockVoxels, clipped to the grid): one
+// workgroup per block, x fastest, so pool accesses are contiguous.
+template <int Kind, bool Color, bool Count>
+__global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FrameParams f, const float4* __restrict__ pix,
+                                                                const uint32_t* __restrict__ color_img,
+                                                                typename VoxelIO<Kind>::Raw* __restrict__ pool,
+                                                                float4* __restrict__ cpool,
+                                                                const uint32_t* __restrict__ block_of_slot,
+                                                                const uint32_t* __restrict__ work,
+                                                                const uint32_t* __restrict__ sp,
+                                                                unsigned long long* __restrict__ fused,
+                                                                uint32_t* __restrict__ obs_bits, BlockMap bm) {
+  __shared__ int any_fused;
+  Rigid w2c;
+  if (!block_w2c(f, w2c)) return;
+  const uint32_t n = sp[1];
+  unsigned int my_fused = 0, my_color = 0;
+  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+    const uint32_t slot = work[li];
+    const uint32_t key = block_of_slot[slot];
+    const int bx = static_cast<int>(key % static_cast<uint32_t>(bm.bx));
+    const int by = static_cast<int>((key / static_cast<uint32_t>(bm.bx)) % static_cast<uint32_t>(bm.by));
+    const int bz = static_cast<int>(key / static_cast<uint32_t>(bm.bx * bm.by));
+    if (threadIdx.x == 0) any_fused = 0;
+    __syncthreads();
+    bool any = false;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int i = static_cast<int>(threadIdx.x) + pass * 256;
+      const int x = bx * 8 + (i & 7), y = by * 8 + ((i >> 3) & 7), z = bz * 8 + (i >> 6);
+      if (x < g.nx && y < g.ny && z < g.nz) {
+        const size_t vi = static_cast<size_t>(slot) * kBlockVoxels + i;
+        any |= fuse_voxel<Kind, Color, Count>(g, f, w2c, pix, color_img, pool + vi, Color ? cpool + vi : nullptr, x,
+                                              y, z, my_fused, my_color);
+      }
+    }
+    if (any) any_fused = 1;
+    __syncthreads();
+    if (threadIdx.x == 0 && any_fused != 0) {
+      const int fb = bm.flat(bx, by, bz);
+      atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
+    }
+    __syncthreads();
+  }
+  if (Count) {
+    for (int off = 32; off > 0; off >>= 1) {
+      my_fused += __shfl_xor(my_fused, off);
+      my_color += __shfl_xor(my_color, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&fused[0], static_cast<unsigned long long>(my_fused));
+      atomicAdd(&fused[1], static_cast<unsigned long long>(my_color));
+    }
+  }
+}
+
+__global__ void fill_table_kernel(unsigned long long* table, uint32_t* stamp, size_t buckets, uint32_t* sp) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < buckets;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    table[i] = kHashEmpty;
+    stamp[i] = 0U;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 8) sp[threadIdx.x] = 0U;
+}
+
+template <int Kind, bool Color>
+void launch_blocks_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
+  using Raw = typename VoxelIO<Kind>::Raw;
+  const int grid = 2048;
+  if (count) {
+    hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
+                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid));
+  } else {
+    hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, false>), dim3(grid), dim3(256), 0, ctx->stream,
+                       ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
+                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid));
+  }
+}
+
+// Dense expansion of a hashed volume (SparseVolumeSnapshot + host_dense_view,
+// block_rep.hpp:170-182,254-272): unallocated voxels read as fresh.
+template <int Kind>
+__global__ void expand_hashed_kernel(Grid g, HashView hv, const typename VoxelIO<Kind>::Raw* __restrict__ pool,
+                                     const float4* __restrict__ cpool, typename VoxelIO<Kind>::Raw* __restrict__ out,
+                                     float4* __restrict__ cout_) {
+  const size_t n = static_cast<size_t>(g.nx) * g.ny * g.nz;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int x = static_cast<int>(i % g.nx), y = static_cast<int>((i / g.nx) % g.ny);
+    const int z = static_cast<int>(i / (static_cast<size_t>(g.nx) * g.ny));
+    const uint32_t slot = hv.find(hv.key_of(x, y, z));
+    const size_t pi = static_cast<size_t>(slot) * kBlockVoxels + intra_of(x, y, z);
+    out[i] = slot == kNoSlot ? VoxelIO<Kind>::fresh() : pool[pi];
+    if (cout_ != nullptr) cout_[i] = slot == kNoSlot ? make_float4(0.0F, 0.0F, 0.0F, 0.0F) : cpool[pi];
+  }
+}
+
 template <int Kind, bool Color>
 void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
   using Raw = typename VoxelIO<Kind>::Raw;
@@ -436,6 +667,40 @@ void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* c
 }
 
 }  // namespace
+
+bool hashed(const kf_context* ctx) { return ctx->cfg.storage == KF_STORAGE_HASHED; }
+
+HashView hash_view(const kf_context* ctx) {
+  const BlockMap bm = block_map(ctx->grid);
+  return HashView{ctx->hash_table, static_cast<uint32_t>(ctx->buckets - 1), bm.bx, bm.by, bm.bz};
+}
+
+void launch_expand_hashed(kf_context* ctx, void* voxels, float4* colors) {
+  const HashView hv = hash_view(ctx);
+  switch (ctx->cfg.voxel_store) {
+    case KF_VOXEL_QUANTIZED:
+      hipLaunchKernelGGL(expand_hashed_kernel<KF_VOXEL_QUANTIZED>, dim3(4096), dim3(256), 0, ctx->stream, ctx->grid,
+                         hv, static_cast<const uint32_t*>(ctx->voxels), ctx->colors, static_cast<uint32_t*>(voxels),
+                         colors);
+      break;
+    case KF_VOXEL_BF16:
+      hipLaunchKernelGGL(expand_hashed_kernel<KF_VOXEL_BF16>, dim3(4096), dim3(256), 0, ctx->stream, ctx->grid, hv,
+                         static_cast<const uint32_t*>(ctx->voxels), ctx->colors, static_cast<uint32_t*>(voxels),
+                         colors);
+      break;
+    default:
+      hipLaunchKernelGGL(expand_hashed_kernel<KF_VOXEL_FLOAT>, dim3(4096), dim3(256), 0, ctx->stream, ctx->grid, hv,
+                         static_cast<const float2*>(ctx->voxels), ctx->colors, static_cast<float2*>(voxels), colors);
+      break;
+  }
+  KF_HIP(hipGetLastError());
+}
+
+void launch_fill_pool(kf_context* ctx) {
+  hipLaunchKernelGGL(fill_table_kernel, dim3(2048), dim3(256), 0, ctx->stream, ctx->hash_table, ctx->hash_stamp,
+                     ctx->buckets, ctx->sparse);
+  KF_HIP(hipGetLastError());
+}
 
 void launch_fill_voxels(kf_context* ctx) {
   const int grid = 4096;
@@ -512,15 +777,48 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
   hipLaunchKernelGGL(pixel_prep_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream,
                      fr.depth, reinterpret_cast<const float4*>(fr.normals), ctx->pix, ctx->tile_max,
                      f, ctx->grid.trunc);
+  const bool color = ctx->colors != nullptr;
+  const uint32_t* cimg = fr.color;
+  const bool timing = c.stage_timing != 0 && from_state;
+  if (hashed(ctx)) {
+    // band walk -> slots and work list -> listed blocks
+    const float step = 0.5F * 8.0F * ctx->grid.voxel;  // TruncationBandWalk::half_block_step
+    const uint32_t limit = static_cast<uint32_t>(ctx->buckets - ctx->buckets / 8);
+    hipLaunchKernelGGL(sparse_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->sparse);
+    hipLaunchKernelGGL(band_walk_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream, ctx->grid, f,
+                       fr.depth, hash_view(ctx), ctx->hash_stamp, ctx->sparse, limit,
+                       static_cast<uint32_t>(ctx->capacity), step);
+    hipLaunchKernelGGL(assign_slots_kernel, dim3(1024), dim3(256), 0, ctx->stream, hash_view(ctx), ctx->hash_stamp,
+                       ctx->sparse, ctx->block_of_slot, ctx->work, static_cast<uint32_t>(ctx->capacity),
+                       static_cast<uint32_t>(ctx->buckets));
+    if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
+    switch (c.voxel_store) {
+      case KF_VOXEL_QUANTIZED:
+        color ? launch_blocks_t<KF_VOXEL_QUANTIZED, true>(ctx, f, cimg, count)
+              : launch_blocks_t<KF_VOXEL_QUANTIZED, false>(ctx, f, cimg, count);
+        break;
+      case KF_VOXEL_BF16:
+        color ? launch_blocks_t<KF_VOXEL_BF16, true>(ctx, f, cimg, count)
+              : launch_blocks_t<KF_VOXEL_BF16, false>(ctx, f, cimg, count);
+        break;
+      default:
+        color ? launch_blocks_t<KF_VOXEL_FLOAT, true>(ctx, f, cimg, count)
+              : launch_blocks_t<KF_VOXEL_FLOAT, false>(ctx, f, cimg, count);
+        break;
+    }
+    if (timing) KF_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
+    launch_skip_map(ctx);
+    if (count && accumulate)
+      hipLaunchKernelGGL(accumulate_counts_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
+    KF_HIP(hipGetLastError());
+    return;
+  }
   hipLaunchKernelGGL(tile_pyramid_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->tile_max, tiles_x, tiles_y);
   const Grid& g = ctx->grid;
   const int nb = ((g.nx + kBrickX - 1) / kBrickX) * ((g.ny + kBrickY - 1) / kBrickY) *
                  ((g.nz + kBrickZ - 1) / kBrickZ);
   hipLaunchKernelGGL(brick_cull_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, g, f,
                      ctx->tile_max, tiles_x, tiles_y, ctx->bricks, ctx->counters);
-  const bool color = ctx->colors != nullptr;
-  const uint32_t* cimg = fr.color;
-  const bool timing = c.stage_timing != 0 && from_state;
   if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
   switch (c.voxel_store) {
     case KF_VOXEL_QUANTIZED:

@@ -210,6 +210,47 @@ struct Grid {
 };
 
 // ---------------------------------------------------------------------------
+// Hashed (sparse) volume: 8^3-voxel blocks in a pool of slots, found through
+// an open-addressed hash table of block keys with linear probing.  A block
+// key is the reference's flat x-major block index (grid.hpp:54-59); voxels
+// of a block sit at (z % 8 * 8 + y % 8) * 8 + x % 8 (grid.hpp:61-69).
+// ---------------------------------------------------------------------------
+constexpr unsigned long long kHashEmpty = ~0ULL;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFU;       // key present, no pool slot (pool exhausted)
+constexpr uint32_t kPendingSlot = 0xFFFFFFFEU;  // key inserted this frame, slot not yet assigned
+constexpr int kBlockVoxels = 512;
+
+__host__ __device__ inline uint32_t intra_of(int x, int y, int z) {
+  return static_cast<uint32_t>((((z & 7) * 8) + (y & 7)) * 8 + (x & 7));
+}
+
+struct HashView {
+  unsigned long long* table;  // (key << 32) | slot, kHashEmpty when free
+  uint32_t mask;              // buckets - 1 (buckets a power of two)
+  int bx, by, bz;             // blocks per axis
+  __host__ __device__ inline uint32_t bucket(uint32_t key) const {
+    return static_cast<uint32_t>((static_cast<unsigned long long>(key) * 0x9E3779B97F4A7C15ULL) >> 32) & mask;
+  }
+  __host__ __device__ inline uint32_t key_of(int x, int y, int z) const {
+    return static_cast<uint32_t>(((z >> 3) * by + (y >> 3)) * bx + (x >> 3));
+  }
+  // Pool slot of a block, or kNoSlot when the block holds no voxels.
+  __device__ inline uint32_t find(uint32_t key) const {
+    uint32_t h = bucket(key);
+    for (uint32_t probe = 0; probe <= mask; ++probe) {
+      const unsigned long long e = table[h];
+      if (e == kHashEmpty) return kNoSlot;
+      if (static_cast<uint32_t>(e >> 32) == key) {
+        const uint32_t slot = static_cast<uint32_t>(e);
+        return slot == kPendingSlot ? kNoSlot : slot;
+      }
+      h = (h + 1) & mask;
+    }
+    return kNoSlot;
+  }
+};
+
+// ---------------------------------------------------------------------------
 // ICP device state, one per context (device memory).
 // ---------------------------------------------------------------------------
 struct IcpState {
@@ -234,6 +275,13 @@ struct IcpState {
   double prev_pose[16];   // camera_to_world of the previous two frames
   double prev2_pose[16];
   long long frames_done;
+  // learned warm start: the constant-velocity seed kept for the fallback,
+  // the network's proposal, and how the proposal fared
+  double seed_pose[16];
+  double proposal[16];
+  double proposal_err_t, proposal_err_r, learned_rms;
+  int used_learned;
+  int pad2;
 };
 
 struct DeviceIcpParams {
@@ -298,12 +346,26 @@ struct kf_context {
   int64_t traj_capacity = 0;
   int64_t frames = 0;
   // timing
-  hipEvent_t ev[14] = {};  // 0 start, 1 pyramid, 2.. per level (raycast, icp), 11 end,
-                          // 12/13 around the voxel-update kernel
+  hipEvent_t ev[15] = {};  // 0 start, 1 pyramid, 2.. per level (raycast, icp), 11 end,
+                          // 12/13 around the voxel-update kernel, 14 after the proposal
   kf_stage_times last_times{};
   bool have_times = false;
-  // learned warm start (pose network), created on demand
-  void* posenet = nullptr;
+  // hashed volume (KF_STORAGE_HASHED): voxels / colors above are the pool
+  unsigned long long* hash_table = nullptr;
+  uint32_t* hash_stamp = nullptr;     // frame id of the last visit, per bucket
+  uint32_t* block_of_slot = nullptr;  // block key of every pool slot
+  uint32_t* work = nullptr;           // pool slots to integrate this frame
+  // [0] slots requested (allocated + overflowing), [1] work count,
+  // [2] table entries, [3] frame id, [4] overflowing requests
+  uint32_t* sparse = nullptr;
+  size_t buckets = 0;
+  size_t capacity = 0;
+  // learned warm start: attached network, previous coarsest-level depth,
+  // the proposal's twist
+  kf_posenet* posenet = nullptr;
+  uint16_t* prev_coarse = nullptr;
+  float* twist = nullptr;
+  bool timed_learned = false;
 };
 
 namespace kf {
@@ -338,6 +400,12 @@ void launch_fill_voxels(kf_context* ctx);
 void launch_skip_map(kf_context* ctx);      // skip bits from observed bits
 void launch_rebuild_observed(kf_context* ctx);  // observed bits from the voxels
 
+bool hashed(const kf_context* ctx);
+HashView hash_view(const kf_context* ctx);
+void launch_fill_pool(kf_context* ctx);  // the table empty, the counters zero
+// the dense expansion of a hashed volume into device buffers of the grid's size
+void launch_expand_hashed(kf_context* ctx, void* voxels, float4* colors);
+
 // Block grid of the occupancy bitmaps: 8^3 voxels per block, x fastest.
 struct BlockMap {
   int bx, by, bz;  // blocks per axis
@@ -348,5 +416,9 @@ inline BlockMap block_map(const Grid& g) {
   return BlockMap{(g.nx + 7) / 8, (g.ny + 7) / 8, (g.nz + 7) / 8};
 }
 DeviceIcpParams to_device_params(const kf_icp_options& o);
+
+// Pose network (kf_posenet.hip): proposal for one depth pair on stream s.
+void posenet_propose(kf_posenet* net, hipStream_t s, const uint16_t* prev, const uint16_t* cur, float* twist);
+bool posenet_input_matches(int width, int height);
 
 }  // namespace kf

@@ -39,8 +39,8 @@ __device__ __forceinline__ double box_exit(const double (&a)[3], const double (&
   return t;
 }
 
-template <int Kind>
-__global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind> vol, RayParams rp, Rigid pose_arg,
+template <int Kind, typename Store>
+__global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, RayParams rp, Rigid pose_arg,
                                                        IcpState* state, float4* __restrict__ pts,
                                                        float4* __restrict__ nrm, uint32_t* __restrict__ col,
                                                        const uint32_t* __restrict__ skip_bits, BlockMap bm) {
@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind> vol, RayPara
       if (ok) {
         P = surface;
         N = n;
-        if (vol.colors != nullptr) {
+        if (vol.s.has_colors()) {
           V3 c;
           if (vol.color(surface, c)) C = pixel_from_color(c);
         } else {
@@ -168,10 +168,19 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind> vol, RayPara
 template <int Kind>
 void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState* state,
               LevelBuffers& L) {
-  Sampler<Kind> vol{static_cast<const typename VoxelIO<Kind>::Raw*>(ctx->voxels), ctx->colors,
-                   ctx->grid, 1.0F / ctx->grid.voxel};
+  using Raw = typename VoxelIO<Kind>::Raw;
   const dim3 grid((rp.w + 15) / 16, (rp.h + 15) / 16);
-  hipLaunchKernelGGL(raycast_kernel<Kind>, grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,
+  if (hashed(ctx)) {
+    Sampler<Kind, HashedStore<Kind>> vol{
+        HashedStore<Kind>{hash_view(ctx), static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
+        1.0F / ctx->grid.voxel};
+    hipLaunchKernelGGL((raycast_kernel<Kind, HashedStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose,
+                       state, L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, block_map(ctx->grid));
+    return;
+  }
+  Sampler<Kind> vol{DenseStore<Kind>{static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
+                    1.0F / ctx->grid.voxel};
+  hipLaunchKernelGGL((raycast_kernel<Kind, DenseStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,
                      L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, block_map(ctx->grid));
 }
 

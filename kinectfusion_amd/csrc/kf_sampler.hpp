@@ -1,5 +1,5 @@
-// Trilinear TSDF sampler over the dense voxel array (reference:
-// TrilinearSampler + DenseCornerAccess, include/kinectfusion/
+// Trilinear TSDF sampler over the dense voxel array or the hashed block pool
+// (reference: TrilinearSampler + DenseCornerAccess / SparseCornerAccess, include/kinectfusion/
 // volume_sampler.hpp:36-255; blend_tsdf / blend_color / cell_gradient /
 // central_normal, trilinear.hpp:85-269).  Shared by the raycaster and the
 // mesher so both read the volume with the same float operations as the CPU
@@ -11,11 +11,70 @@
 
 namespace kf {
 
+// Corner access (DenseCornerAccess / SparseCornerAccess,
+// volume_sampler.hpp:36-109 and block_rep.hpp:113-160).  A voxel of an
+// unallocated block reads as a fresh voxel and a colour voxel as zero: the
+// blends treat both exactly like the reference's missing (null) corners.
+// The last block a hashed store looked up (key, pool slot).
+struct SlotCache {
+  uint32_t key = kNoSlot;
+  uint32_t slot = kNoSlot;
+};
+
 template <int Kind>
+struct DenseStore {
+  using Raw = typename VoxelIO<Kind>::Raw;
+  const Raw* __restrict__ vox;
+  const float4* __restrict__ colors;
+  __device__ inline Raw voxel(const Grid& g, int x, int y, int z) const { return vox[g.flat(x, y, z)]; }
+  __device__ inline Raw voxel_cached(const Grid& g, int x, int y, int z, SlotCache&) const {
+    return vox[g.flat(x, y, z)];
+  }
+  __device__ inline float4 color(const Grid& g, int x, int y, int z) const { return colors[g.flat(x, y, z)]; }
+  __device__ inline float4 color_cached(const Grid& g, int x, int y, int z, SlotCache&) const {
+    return colors[g.flat(x, y, z)];
+  }
+  __device__ inline bool has_colors() const { return colors != nullptr; }
+};
+
+template <int Kind>
+struct HashedStore {
+  using Raw = typename VoxelIO<Kind>::Raw;
+  HashView hv;
+  const Raw* __restrict__ pool;
+  const float4* __restrict__ cpool;
+  __device__ inline uint32_t slot_of(int x, int y, int z, SlotCache& cache) const {
+    const uint32_t key = hv.key_of(x, y, z);
+    if (key != cache.key) {
+      cache.key = key;
+      cache.slot = hv.find(key);
+    }
+    return cache.slot;
+  }
+  __device__ inline Raw voxel_cached(const Grid&, int x, int y, int z, SlotCache& cache) const {
+    const uint32_t slot = slot_of(x, y, z, cache);
+    return slot == kNoSlot ? VoxelIO<Kind>::fresh() : pool[static_cast<size_t>(slot) * kBlockVoxels + intra_of(x, y, z)];
+  }
+  __device__ inline Raw voxel(const Grid& g, int x, int y, int z) const {
+    SlotCache cache;
+    return voxel_cached(g, x, y, z, cache);
+  }
+  __device__ inline float4 color_cached(const Grid&, int x, int y, int z, SlotCache& cache) const {
+    const uint32_t slot = slot_of(x, y, z, cache);
+    return slot == kNoSlot ? make_float4(0.0F, 0.0F, 0.0F, 0.0F)
+                           : cpool[static_cast<size_t>(slot) * kBlockVoxels + intra_of(x, y, z)];
+  }
+  __device__ inline float4 color(const Grid& g, int x, int y, int z) const {
+    SlotCache cache;
+    return color_cached(g, x, y, z, cache);
+  }
+  __device__ inline bool has_colors() const { return cpool != nullptr; }
+};
+
+template <int Kind, typename Store = DenseStore<Kind>>
 struct Sampler {
   using IO = VoxelIO<Kind>;
-  const typename IO::Raw* __restrict__ vox;
-  const float4* __restrict__ colors;
+  Store s;
   Grid g;
   float inv_voxel;
 
@@ -40,6 +99,7 @@ struct Sampler {
                           bz + 1 < g.nz;
     float acc = 0.0F, wsum = 0.0F;
     int valid = 0;
+    SlotCache cache;
 #pragma unroll
     for (int oz = 0; oz <= 1; ++oz)
 #pragma unroll
@@ -50,7 +110,7 @@ struct Sampler {
           if (!interior && (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz)) continue;
           float t, w;
           bool obs, fin;
-          IO::unpack(vox[g.flat(x, y, z)], t, w, obs, fin);
+          IO::unpack(s.voxel_cached(g, x, y, z, cache), t, w, obs, fin);
           if (!obs || !fin) continue;
           const float cw = axis_w(fr.x, ox) * axis_w(fr.y, oy) * axis_w(fr.z, oz);
           acc += cw * t;
@@ -67,7 +127,7 @@ struct Sampler {
 
   // blend_color.
   __device__ bool color(V3 p, V3& out) const {
-    if (colors == nullptr) return false;
+    if (!s.has_colors()) return false;
     const V3 gc = grid_coords(p);
     if (!contains(gc)) return false;
     const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
@@ -76,6 +136,7 @@ struct Sampler {
     const V3 fr = sub3(gg, fl);
     V3 acc = mk3(0.0F, 0.0F, 0.0F);
     float wsum = 0.0F;
+    SlotCache cache;
 #pragma unroll
     for (int oz = 0; oz <= 1; ++oz)
 #pragma unroll
@@ -84,7 +145,7 @@ struct Sampler {
         for (int ox = 0; ox <= 1; ++ox) {
           const int x = bx + ox, y = by + oy, z = bz + oz;
           if (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz) continue;
-          const float4 c = colors[g.flat(x, y, z)];
+          const float4 c = s.color_cached(g, x, y, z, cache);
           if (c.w <= 0.0F) continue;
           const float cw = axis_w(fr.x, ox) * axis_w(fr.y, oy) * axis_w(fr.z, oz);
           acc = add3(acc, scale3(mk3(c.x, c.y, c.z), cw));
@@ -125,6 +186,7 @@ struct Sampler {
     const V3 f = sub3(gg, fl);
     float c[2][2][2];
     bool complete = true;
+    SlotCache cache;
 #pragma unroll
     for (int oz = 0; oz <= 1; ++oz)
 #pragma unroll
@@ -139,7 +201,7 @@ struct Sampler {
           }
           float t, w;
           bool obs, fin;
-          IO::unpack(vox[g.flat(x, y, z)], t, w, obs, fin);
+          IO::unpack(s.voxel_cached(g, x, y, z, cache), t, w, obs, fin);
           if (!obs || !fin) complete = false;
           c[oz][oy][ox] = t;
         }

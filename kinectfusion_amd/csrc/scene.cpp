@@ -11,33 +11,28 @@
 //
 // Randomness is counter-based (SplitMix64 of seed/frame/pixel) with
 // Box-Muller normals, so any frame renders identically on any host and in
-// any order.
+// any order.  The per-pixel renderer lives in scene_geometry.hpp, shared with
+// the device renderer.
+
+#define KF_SCENE_FN inline
+#include "scene_geometry.hpp"
 
 #include <algorithm>
-#include <cmath>
-#include <cstdint>
 #include <cstring>
-#include <vector>
 
 namespace {
 
-struct D3 {
-  double x, y, z;
-};
-inline D3 operator+(D3 a, D3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline D3 operator-(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline D3 operator*(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+using kfscene::Box;
+using kfscene::D3;
+using kfscene::SceneDesc;
+using kfscene::Sphere;
+using kfscene::splitmix64;
+using kfscene::unit;
 
-inline std::uint64_t splitmix64(std::uint64_t x) {
-  x += 0x9E3779B97F4A7C15ULL;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
-  return x ^ (x >> 31);
-}
-inline double unit(std::uint64_t bits) {  // [0, 1) with 53 bits
-  return static_cast<double>(bits >> 11) * (1.0 / 9007199254740992.0);
-}
+inline D3 operator+(D3 a, D3 b) { return kfscene::add(a, b); }
+inline D3 operator-(D3 a, D3 b) { return kfscene::sub(a, b); }
+inline D3 operator*(D3 a, double s) { return kfscene::scale(a, s); }
+using kfscene::dot;
 
 struct Rng {
   std::uint64_t state;
@@ -45,90 +40,10 @@ struct Rng {
   double uniform(double lo, double hi) { return lo + (hi - lo) * unit(next()); }
 };
 
-struct Sphere {
-  D3 c;
-  double r;
-  std::uint32_t rgba;
-};
-struct Box {
-  D3 lo, hi;
-  std::uint32_t rgba;
-};
-
-struct Scene {
-  int kind;
-  std::uint64_t seed;
-  int width, height;
-  double fx, fy, cx, cy;
-  D3 room_lo, room_hi;
-  std::uint32_t wall_rgba[6];
-  std::vector<Sphere> spheres;
-  std::vector<Box> boxes;
-  // trajectory parameters
-  double t_amp[3], t_freq[3], t_phase[3];
-  double r_amp[3], r_freq[3], r_phase[3];
-  double pan_rate;  // radians per frame about the vertical axis (hall sweep)
-  double walk_rate; // metres per frame along x (hall sweep)
-};
-
 std::uint32_t random_rgba(Rng& rng) {
   const auto c = [&]() { return static_cast<std::uint32_t>(rng.uniform(40.0, 250.0)); };
   const std::uint32_t r = c(), g = c(), b = c();
   return r | (g << 8U) | (b << 16U) | (0xFFU << 24U);
-}
-
-// Ray/sphere: smallest s > eps with |o + s d - c| = r.
-inline bool hit_sphere(const Sphere& s, D3 o, D3 d, double& t) {
-  const D3 oc = o - s.c;
-  const double b = dot(oc, d), c = dot(oc, oc) - s.r * s.r, a = dot(d, d);
-  const double disc = b * b - a * c;
-  if (disc < 0.0) return false;
-  const double q = std::sqrt(disc);
-  double s0 = (-b - q) / a;
-  if (s0 <= 1e-9) s0 = (-b + q) / a;
-  if (s0 <= 1e-9) return false;
-  t = s0;
-  return true;
-}
-// Ray/box (outside surface): slab entry.
-inline bool hit_box(const Box& bx, D3 o, D3 d, double& t) {
-  double tmin = -1e300, tmax = 1e300;
-  const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
-  const double lo[3] = {bx.lo.x, bx.lo.y, bx.lo.z}, hi[3] = {bx.hi.x, bx.hi.y, bx.hi.z};
-  for (int a = 0; a < 3; ++a) {
-    if (std::fabs(dd[a]) < 1e-15) {
-      if (oo[a] < lo[a] || oo[a] > hi[a]) return false;
-      continue;
-    }
-    double t0 = (lo[a] - oo[a]) / dd[a], t1 = (hi[a] - oo[a]) / dd[a];
-    if (t0 > t1) std::swap(t0, t1);
-    tmin = std::max(tmin, t0);
-    tmax = std::min(tmax, t1);
-  }
-  if (tmax < tmin || tmin <= 1e-9) return false;
-  t = tmin;
-  return true;
-}
-// Ray from inside the room to its walls: exit distance and wall index.
-inline bool hit_room(const Scene& sc, D3 o, D3 d, double& t, int& wall) {
-  const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
-  const double lo[3] = {sc.room_lo.x, sc.room_lo.y, sc.room_lo.z};
-  const double hi[3] = {sc.room_hi.x, sc.room_hi.y, sc.room_hi.z};
-  double best = 1e300;
-  int bw = -1;
-  for (int a = 0; a < 3; ++a) {
-    if (dd[a] > 1e-15) {
-      const double s = (hi[a] - oo[a]) / dd[a];
-      if (s > 1e-9 && s < best) { best = s; bw = 2 * a + 1; }
-    } else if (dd[a] < -1e-15) {
-      const double s = (lo[a] - oo[a]) / dd[a];
-      if (s > 1e-9 && s < best) { best = s; bw = 2 * a; }
-    }
-  }
-  if (bw < 0) return false;
-  t = best;
-  wall = bw;
-  return true;
 }
 
 void euler_to_rot(double rx, double ry, double rz, double* R) {
@@ -157,7 +72,8 @@ extern "C" {
 //                    spheres; the camera pans and walks along x.
 void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, double fx, double fy,
                       double cx, double cy) {
-  auto* sc = new Scene();
+  auto* sc = new SceneDesc();
+  std::memset(sc, 0, sizeof(SceneDesc));
   sc->kind = kind;
   sc->seed = seed;
   sc->width = width;
@@ -184,7 +100,7 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
       s.c.x = std::clamp(s.c.x, sc->room_lo.x + s.r + 0.02, sc->room_hi.x - s.r - 0.02);
       s.c.y = std::clamp(s.c.y, sc->room_lo.y + s.r + 0.02, sc->room_hi.y - s.r - 0.02);
       s.rgba = random_rgba(rng);
-      sc->spheres.push_back(s);
+      sc->spheres[sc->n_spheres++] = s;
     }
     // per-frame translation <= sqrt(3) * 0.0055 m, rotation <= 3 * 0.16 deg
     for (int a = 0; a < 3; ++a) {
@@ -216,13 +132,13 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
         b.lo = c - h;
         b.hi = c + h;
         b.rgba = random_rgba(rng);
-        sc->boxes.push_back(b);
+        sc->boxes[sc->n_boxes++] = b;
       } else {
         Sphere s{};
         s.c = c;
         s.r = rng.uniform(0.1, 0.35);
         s.rgba = random_rgba(rng);
-        sc->spheres.push_back(s);
+        sc->spheres[sc->n_spheres++] = s;
       }
     }
     for (int a = 0; a < 3; ++a) {
@@ -239,12 +155,12 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
   return sc;
 }
 
-void kf_scene_destroy(void* s) { delete static_cast<Scene*>(s); }
+void kf_scene_destroy(void* s) { delete static_cast<SceneDesc*>(s); }
 
 // Ground-truth camera_to_world of frame k (row-major 4x4, FP64); frame 0 is
 // the identity.
 void kf_scene_pose(const void* s, int frame, double* pose) {
-  const Scene& sc = *static_cast<const Scene*>(s);
+  const SceneDesc& sc = *static_cast<const SceneDesc*>(s);
   const double k = frame;
   double t[3], e[3];
   for (int a = 0; a < 3; ++a) {
@@ -272,61 +188,27 @@ void kf_scene_pose(const void* s, int frame, double* pose) {
 // (use the frame index); noise == 0 renders the exact depth, rounded to mm.
 void kf_scene_render(const void* s, const double* pose, std::uint64_t noise_key, int noise,
                      std::uint16_t* depth_mm, std::uint32_t* rgba, float* true_depth) {
-  const Scene& sc = *static_cast<const Scene*>(s);
-  const D3 o{pose[3], pose[7], pose[11]};
+  const SceneDesc& sc = *static_cast<const SceneDesc*>(s);
   const int W = sc.width, H = sc.height;
 #pragma omp parallel for schedule(static)
   for (int v = 0; v < H; ++v) {
     for (int u = 0; u < W; ++u) {
-      const D3 dc{(u - sc.cx) / sc.fx, (v - sc.cy) / sc.fy, 1.0};
-      const D3 d{pose[0] * dc.x + pose[1] * dc.y + pose[2] * dc.z,
-                 pose[4] * dc.x + pose[5] * dc.y + pose[6] * dc.z,
-                 pose[8] * dc.x + pose[9] * dc.y + pose[10] * dc.z};
-      double best = 1e300;
-      std::uint32_t col = 0;
-      int wall = -1;
-      double tw;
-      if (hit_room(sc, o, d, tw, wall)) {
-        best = tw;
-        col = sc.wall_rgba[wall];
-      }
-      for (const Sphere& sp : sc.spheres) {
-        double t;
-        if (hit_sphere(sp, o, d, t) && t < best) {
-          best = t;
-          col = sp.rgba;
-        }
-      }
-      for (const Box& bx : sc.boxes) {
-        double t;
-        if (hit_box(bx, o, d, t) && t < best) {
-          best = t;
-          col = bx.rgba;
-        }
-      }
       const std::size_t i = static_cast<std::size_t>(v) * W + u;
-      // camera-frame z of the hit equals the ray parameter (dc.z == 1)
-      const double z = best < 1e299 ? best : 0.0;
+      std::uint16_t d = 0;
+      std::uint32_t c = 0;
+      double z = 0.0;
+      kfscene::render_pixel(sc, pose, noise_key, noise, u, v, d, c, z);
+      depth_mm[i] = d;
+      if (rgba != nullptr) rgba[i] = c;
       if (true_depth != nullptr) true_depth[i] = static_cast<float>(z);
-      double zn = z;
-      std::uint64_t key = splitmix64(sc.seed ^ (noise_key * 0xD1B54A32D192ED03ULL) ^
-                                     (static_cast<std::uint64_t>(i) * 0x9E3779B97F4A7C15ULL));
-      bool drop = false;
-      if (noise != 0 && z > 0.0) {
-        const std::uint64_t a = splitmix64(key), b = splitmix64(a), c = splitmix64(b);
-        const double u1 = 1.0 - unit(a), u2 = unit(b);  // u1 in (0, 1]
-        const double gauss = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-        const double sigma = 0.0012 + 0.0019 * (z - 0.4) * (z - 0.4);
-        zn = z + sigma * gauss;
-        drop = unit(c) < 0.01;
-      }
-      double mm = std::floor(zn * 1000.0 + 0.5);
-      if (!(mm > 0.0) || z <= 0.0 || drop) mm = 0.0;
-      if (mm > 65535.0) mm = 0.0;
-      depth_mm[i] = static_cast<std::uint16_t>(mm);
-      if (rgba != nullptr) rgba[i] = z > 0.0 ? col : 0U;
     }
   }
+}
+
+// The plain scene description (kfscene::SceneDesc) for the device renderer.
+const void* kf_scene_description(const void* s, std::size_t* bytes) {
+  if (bytes != nullptr) *bytes = sizeof(SceneDesc);
+  return s;
 }
 
 }  // extern "C"

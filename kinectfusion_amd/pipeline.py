@@ -116,6 +116,14 @@ class PipelineConfig:
     icp_iterations: tuple[int, ...] = (10, 5, 4)
     stage_timing: bool = False
     count_fused: bool = False
+    # hashed storage (BlockRep, block_rep.hpp:183-307)
+    hash_buckets: int = 1 << 20
+    block_capacity: int = 500_000
+    # learned warm start: iterations per level from the network's proposal
+    # (0 skips a level) and the final RMS (m) above which the frame falls
+    # back to the constant-velocity seed with icp_iterations
+    learned_iterations: tuple[int, ...] = (4, 0, 0)
+    learned_max_rms: float = 0.01
 
     def native(self) -> N.KfConfig:
         if self.space != "hip":
@@ -153,6 +161,10 @@ class PipelineConfig:
         c.icp_iterations[:] = its[:4]
         c.stage_timing = int(self.stage_timing)
         c.count_fused = int(self.count_fused)
+        c.hash_buckets, c.block_capacity = self.hash_buckets, self.block_capacity
+        lits = list(self.learned_iterations) + [0] * (4 - len(self.learned_iterations))
+        c.learned_iterations[:] = lits[:4]
+        c.learned_max_rms = self.learned_max_rms
         return c
 
 
@@ -350,6 +362,26 @@ class Pipeline:
         t = N.KfStageTimes()
         self._check(N.hip().kf_last_stage_times(self._ctx, C.byref(t)))
         return t
+
+    def attach_posenet(self, net) -> None:
+        """Use a trained PoseNet for init="learned" frames (None detaches)."""
+        self._check(N.hip().kf_attach_posenet(self._ctx, net.handle if net is not None else None))
+        self._posenet = net
+
+    def hashed_stats(self) -> dict:
+        """Hashed volume: allocated blocks, dropped requests (overflow), hash
+        entries, blocks integrated by the last frame."""
+        out = (C.c_int64 * 4)()
+        self._check(N.hip().kf_hashed_stats(self._ctx, out))
+        return {"allocated": out[0], "overflow": out[1], "entries": out[2], "last_work": out[3]}
+
+    def hashed_blocks(self) -> np.ndarray:
+        """Sorted keys of the allocated blocks (flat x-major block indices)."""
+        n = C.c_int64()
+        self._check(N.hip().kf_hashed_blocks(self._ctx, None, 0, C.byref(n)))
+        out = np.empty(max(n.value, 1), np.uint32)
+        self._check(N.hip().kf_hashed_blocks(self._ctx, out.ctypes.data_as(C.c_void_p), n.value, C.byref(n)))
+        return out[: n.value]
 
     def fused_counts(self) -> tuple[int, int]:
         out = (C.c_int64 * 2)()

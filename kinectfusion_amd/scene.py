@@ -45,6 +45,25 @@ class SyntheticScene:
                                   exact.ctypes.data_as(C.c_void_p) if exact is not None else None)
         return (depth, rgba, exact) if true_depth else (depth, rgba)
 
+    def render_device(self, poses: np.ndarray, keys, noise: bool = True):
+        """Render len(poses) views on the GPU (same per-pixel renderer as
+        render()); returns a (n, H, W) int16 cuda tensor of u16 depth."""
+        import torch
+
+        poses = np.ascontiguousarray(np.asarray(poses, np.float64).reshape(-1, 16))
+        keys = np.ascontiguousarray(np.asarray(keys, np.uint64).reshape(-1))
+        n = poses.shape[0]
+        if keys.shape[0] != n:
+            raise ValueError("one noise key per pose")
+        torch.cuda.synchronize()
+        out = torch.empty((n, self.height, self.width), dtype=torch.int16, device="cuda")
+        rc = self._lib.kf_scene_render_device(self._h, poses.ctypes.data_as(C.POINTER(C.c_double)),
+                                              keys.ctypes.data_as(C.c_void_p), n, int(noise), out.data_ptr(),
+                                              None, None)
+        if rc != 0:
+            raise RuntimeError(f"device render failed with HIP error {rc}")
+        return out
+
     def frame(self, index: int, noise: bool = True):
         """(depth, rgba, pose) of trajectory frame `index`."""
         pose = self.pose(index)

@@ -445,11 +445,65 @@ void kfo_volume_reset(const kfo_geometry* g, int kind, void* voxels, float* colo
   if (colors != nullptr) std::memset(colors, 0, n * 4 * sizeof(float));
 }
 
+// IntegrationContext::observe/fuse (tsdf_integration.hpp:183-252) with the
+// Classic/AngleWeighted rules (:287-342) for one voxel.  Returns 0 when the
+// voxel is not fused, 1 when fused, 2 when fused with colour.
+static int integrate_voxel(Volume& vol, std::int64_t x, std::int64_t y, std::int64_t z, int rule,
+                           const Rigid& w2c, const Intr& intr, const std::uint16_t* depth, int width,
+                           int height, const std::uint32_t* color, const float* normals,
+                           const kfo_integration_options* opt) {
+  // observe (tsdf_integration.hpp:184-224)
+  const V3 cp = apply(w2c, vol.cell_center(x, y, z));
+  if (cp.z <= 0.0F) return 0;  // :150-152
+  float u, v;
+  project(intr, cp, u, v);
+  long col, row;
+  if (!round_index(u, width, col) || !round_index(v, height, row)) return 0;
+  float d;
+  if (!depth_in_range(depth[row * width + col], opt->depth_scale, opt->min_depth, opt->max_depth, d)) return 0;
+  const V3 ray = back_project(intr, static_cast<float>(col), static_cast<float>(row), 1.0F);
+  const float lambda = norm(ray);
+  if (!all_finite(ray) || lambda == 0.0F) return 0;
+  const float trunc = opt->distance_scaled_truncation ? vol.truncation + (opt->truncation_distance_scale * d)
+                                                      : vol.truncation;
+  const float sd = opt->projective_distance ? d - (norm(cp) / lambda) : d - cp.z;
+  if (sd < -trunc) return 0;
+  const float tsdf = cclamp(sd / trunc, -1.0F, 1.0F);
+  const bool integrate_color = sd <= trunc * 0.5F;
+  // weights (:287-342)
+  float w = opt->observation_weight;
+  if (rule == 1 && normals != nullptr) {
+    const float* n = normals + 3 * (static_cast<std::size_t>(row) * width + col);
+    const V3 ns = mk(n[0], n[1], n[2]);
+    if (all_finite(ns)) {
+      const V3 view = -normalized(ray);
+      const float cos_theta = cmax(0.0F, dot(normalized(ns), view));
+      w *= cos_theta / d;
+    }
+  }
+  // fuse (:230-252)
+  if (w <= 0.0F) return 0;
+  const std::size_t i = vol.flat(x, y, z);
+  vol.fuse(i, tsdf, w, opt->max_weight);
+  if (vol.colors != nullptr && color != nullptr && integrate_color) {
+    const V3 c = color_from_pixel(color[static_cast<std::size_t>(row) * width + col]);
+    float* cv = vol.colors + 4 * i;
+    const V3 old = mk(cv[0], cv[1], cv[2]);
+    const float cw = cv[3];
+    const V3 avg = ((old * cw) + (c * w)) / (cw + w);  // vector.hpp:233-239
+    cv[0] = avg.x;
+    cv[1] = avg.y;
+    cv[2] = avg.z;
+    cv[3] = cmin(cw + w, opt->max_weight);
+    return 2;
+  }
+  return 1;
+}
+
 // Dense full-sweep integration of one frame: IntegrationSweep<kHost>::run
-// (tsdf_integration.hpp:356-370) with IntegrationContext::observe/fuse
-// (:183-252) and the Classic/AngleWeighted rules (:287-342).
-// rule: 0 classic, 1 angle-weighted.  world_to_camera: 3x4 row-major float.
-// counts[0] = voxels fused with w > 0, counts[1] = colour voxels fused.
+// (tsdf_integration.hpp:356-370).  rule: 0 classic, 1 angle-weighted.
+// world_to_camera: 3x4 row-major float.  counts[0] = voxels fused with w > 0,
+// counts[1] = colour voxels fused.
 int kfo_integrate(const kfo_geometry* g, int kind, int rule, void* voxels, float* colors,
                   const std::uint16_t* depth, int width, int height, const std::uint32_t* color,
                   const float* normals, const kfo_intrinsics* k, const float* world_to_camera,
@@ -461,59 +515,146 @@ int kfo_integrate(const kfo_geometry* g, int kind, int rule, void* voxels, float
   for (std::int64_t z = 0; z < vol.nz; ++z)
     for (std::int64_t y = 0; y < vol.ny; ++y)
       for (std::int64_t x = 0; x < vol.nx; ++x) {
-        // observe (tsdf_integration.hpp:184-224)
-        const V3 cp = apply(w2c, vol.cell_center(x, y, z));
-        if (cp.z <= 0.0F) continue;  // :150-152
-        float u, v;
-        project(intr, cp, u, v);
-        long col, row;
-        if (!round_index(u, width, col) || !round_index(v, height, row)) continue;
-        float d;
-        if (!depth_in_range(depth[row * width + col], opt->depth_scale, opt->min_depth,
-                            opt->max_depth, d))
-          continue;
-        const V3 ray = back_project(intr, static_cast<float>(col), static_cast<float>(row), 1.0F);
-        const float lambda = norm(ray);
-        if (!all_finite(ray) || lambda == 0.0F) continue;
-        const float trunc = opt->distance_scaled_truncation
-                                ? vol.truncation + (opt->truncation_distance_scale * d)
-                                : vol.truncation;
-        const float sd = opt->projective_distance ? d - (norm(cp) / lambda) : d - cp.z;
-        if (sd < -trunc) continue;
-        const float tsdf = cclamp(sd / trunc, -1.0F, 1.0F);
-        const bool integrate_color = sd <= trunc * 0.5F;
-        // weights (:287-342)
-        float w = opt->observation_weight;
-        if (rule == 1 && normals != nullptr) {
-          const float* n = normals + 3 * (static_cast<std::size_t>(row) * width + col);
-          const V3 ns = mk(n[0], n[1], n[2]);
-          if (all_finite(ns)) {
-            const V3 view = -normalized(ray);
-            const float cos_theta = cmax(0.0F, dot(normalized(ns), view));
-            w *= cos_theta / d;
-          }
-        }
-        // fuse (:230-252)
-        if (w <= 0.0F) continue;
-        const std::size_t i = vol.flat(x, y, z);
-        vol.fuse(i, tsdf, w, opt->max_weight);
-        ++fused;
-        if (colors != nullptr && color != nullptr && integrate_color) {
-          const V3 c = color_from_pixel(color[static_cast<std::size_t>(row) * width + col]);
-          float* cv = colors + 4 * i;
-          const V3 old = mk(cv[0], cv[1], cv[2]);
-          const float cw = cv[3];
-          const V3 avg = ((old * cw) + (c * w)) / (cw + w);  // vector.hpp:233-239
-          cv[0] = avg.x;
-          cv[1] = avg.y;
-          cv[2] = avg.z;
-          cv[3] = cmin(cw + w, opt->max_weight);
-          ++color_fused;
-        }
+        const int r = integrate_voxel(vol, x, y, z, rule, w2c, intr, depth, width, height, color, normals, opt);
+        fused += r > 0 ? 1 : 0;
+        color_fused += r > 1 ? 1 : 0;
       }
   if (counts != nullptr) {
     counts[0] = fused;
     counts[1] = color_fused;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse (block) volume: BlockRep<kHost> (block_rep.hpp:183-307) driven by
+// TruncationBandWalk (tsdf_integration.hpp:378-437).  The volume is kept as
+// its dense expansion (an unallocated block reads as fresh voxels, which every
+// reader treats like the reference's missing corners: trilinear.hpp:150-160)
+// plus the reference's slot grid, so readers compare one to one; the pool
+// layout itself is not observable through the reference's interfaces.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr std::int64_t kBlockEdge = 8;  // grid.hpp:15-17
+struct BlockDims {
+  std::int64_t bx, by, bz;
+  std::int64_t count() const { return bx * by * bz; }
+  std::int64_t of_voxel(std::int64_t x, std::int64_t y, std::int64_t z) const {  // grid.hpp:54-59
+    return ((z / kBlockEdge) * by + (y / kBlockEdge)) * bx + (x / kBlockEdge);
+  }
+};
+BlockDims block_dims(const Volume& v) {  // grid.hpp:27-34
+  return BlockDims{(v.nx + kBlockEdge - 1) / kBlockEdge, (v.ny + kBlockEdge - 1) / kBlockEdge,
+                   (v.nz + kBlockEdge - 1) / kBlockEdge};
+}
+// RigidTransform inverse: {R^T, -R^T t} (vector.hpp:214-224).
+Rigid rigid_inverse(const Rigid& m) {
+  Rigid r{};
+  r.rx = mk(m.rx.x, m.ry.x, m.rz.x);
+  r.ry = mk(m.rx.y, m.ry.y, m.rz.y);
+  r.rz = mk(m.rx.z, m.ry.z, m.rz.z);
+  r.t = -rot(r, m.t);
+  return r;
+}
+}  // namespace
+
+// TruncationBandWalk::visit_pixel (tsdf_integration.hpp:384-428) over every
+// pixel in row-major order: the visited block indices in visit order (each
+// pixel skips a block equal to its previous visit).  Returns the number of
+// visits; writes at most max_visits of them.
+std::int64_t kfo_band_walk(const kfo_geometry* g, const std::uint16_t* depth, int width, int height,
+                           const kfo_intrinsics* k, const float* world_to_camera,
+                           const kfo_integration_options* opt, float step, std::int64_t* visits,
+                           std::int64_t max_visits) {
+  Volume vol = make_volume(g, kFloatVoxel, nullptr, nullptr);
+  const BlockDims blocks = block_dims(vol);
+  const Rigid c2w = rigid_inverse(rigid_from(world_to_camera));
+  const Intr intr{k->fx, k->fy, k->cx, k->cy};
+  const float inv_voxel = 1.0F / vol.voxel_size;
+  std::int64_t n = 0;
+  for (int row = 0; row < height; ++row)
+    for (int col = 0; col < width; ++col) {
+      float measured;
+      if (!depth_in_range(depth[static_cast<std::size_t>(row) * width + col], opt->depth_scale, opt->min_depth,
+                          opt->max_depth, measured))
+        continue;
+      const float band = opt->distance_scaled_truncation
+                             ? vol.truncation + (opt->truncation_distance_scale * measured)
+                             : vol.truncation;
+      const int samples = static_cast<int>((2.0F * band) / step) + 1;
+      std::int64_t previous = -1;
+      for (int s = 0; s <= samples; ++s) {
+        const float d = (measured - band) + (static_cast<float>(s) * step);
+        const V3 world = apply(c2w, back_project(intr, static_cast<float>(col), static_cast<float>(row), d));
+        const V3 grid = (world - vol.origin) * inv_voxel;
+        if (!vol.contains(grid)) continue;
+        const std::int64_t b = blocks.of_voxel(static_cast<std::int64_t>(grid.x), static_cast<std::int64_t>(grid.y),
+                                               static_cast<std::int64_t>(grid.z));
+        if (b == previous) continue;
+        previous = b;
+        if (n < max_visits) visits[n] = b;
+        ++n;
+      }
+    }
+  return n;
+}
+
+// BlockRep<kHost>::integrate (block_rep.hpp:244-307): band walk at half a
+// block, allocation in walk order (slots[block] = pool slot or -1;
+// state[0] = blocks allocated so far including overflowing requests,
+// state[1] = capacity), the frame's work list once per block, then the
+// listed blocks' voxels updated as in the dense sweep.  counts as in
+// kfo_integrate plus counts[2] = blocks in the work list.
+int kfo_integrate_sparse(const kfo_geometry* g, int kind, int rule, void* voxels, float* colors,
+                         std::int32_t* slots, std::int64_t* state, const std::uint16_t* depth, int width,
+                         int height, const std::uint32_t* color, const float* normals, const kfo_intrinsics* k,
+                         const float* world_to_camera, const kfo_integration_options* opt, std::int64_t* counts) {
+  Volume vol = make_volume(g, kind, voxels, colors);
+  const BlockDims blocks = block_dims(vol);
+  const float step = 0.5F * static_cast<float>(kBlockEdge) * vol.voxel_size;  // :380-383
+  std::vector<std::int64_t> visits(static_cast<std::size_t>(width) * height * 8);
+  std::int64_t n = kfo_band_walk(g, depth, width, height, k, world_to_camera, opt, step, visits.data(),
+                                 static_cast<std::int64_t>(visits.size()));
+  if (n > static_cast<std::int64_t>(visits.size())) {
+    visits.resize(static_cast<std::size_t>(n));
+    n = kfo_band_walk(g, depth, width, height, k, world_to_camera, opt, step, visits.data(), n);
+  }
+  std::vector<std::uint8_t> marked(static_cast<std::size_t>(blocks.count()), 0);
+  std::vector<std::int64_t> work;
+  for (std::int64_t i = 0; i < n; ++i) {  // allocate (block_rep.hpp:284-301)
+    const std::int64_t b = visits[static_cast<std::size_t>(i)];
+    if (slots[b] >= 0) {
+      if (marked[b] == 0) {
+        marked[b] = 1;
+        work.push_back(b);
+      }
+      continue;
+    }
+    if (state[0] >= state[1]) {
+      ++state[0];  // records the overflow
+      continue;
+    }
+    slots[b] = static_cast<std::int32_t>(state[0]++);
+    marked[b] = 1;
+    work.push_back(b);
+  }
+  const Rigid w2c = rigid_from(world_to_camera);
+  const Intr intr{k->fx, k->fy, k->cx, k->cy};
+  std::int64_t fused = 0, color_fused = 0;
+  for (const std::int64_t b : work) {  // BlockVoxels (grid.hpp:230-274)
+    const std::int64_t bx = b % blocks.bx, by = (b / blocks.bx) % blocks.by, bz = b / (blocks.bx * blocks.by);
+    for (std::int64_t z = bz * kBlockEdge; z < cmin(vol.nz, (bz + 1) * kBlockEdge); ++z)
+      for (std::int64_t y = by * kBlockEdge; y < cmin(vol.ny, (by + 1) * kBlockEdge); ++y)
+        for (std::int64_t x = bx * kBlockEdge; x < cmin(vol.nx, (bx + 1) * kBlockEdge); ++x) {
+          const int r = integrate_voxel(vol, x, y, z, rule, w2c, intr, depth, width, height, color, normals, opt);
+          fused += r > 0 ? 1 : 0;
+          color_fused += r > 1 ? 1 : 0;
+        }
+  }
+  if (counts != nullptr) {
+    counts[0] = fused;
+    counts[1] = color_fused;
+    counts[2] = static_cast<std::int64_t>(work.size());
   }
   return 0;
 }
@@ -1092,27 +1233,28 @@ constexpr int kEdge[12][3] = {{0, 1, 0}, {1, 2, 1}, {3, 2, 0}, {0, 3, 1}, {4, 5,
 int edge_of(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
 }  // namespace
 
-void* kfo_mesh_extract(const kfo_geometry* g, int kind, const void* voxels, const float* colors,
-                       float min_weight) {
+// MarchingCubes::extract (marching_cubes.cpp:478-507).  slots == nullptr:
+// dense traversal, cells x-fastest; otherwise the sparse traversal: blocks
+// with slots[block] >= 0 in flat order, each block's cells x-fastest within
+// the cell grid (BlockVoxels over (N - 1)^3).
+void* kfo_mesh_extract_traversal(const kfo_geometry* g, int kind, const void* voxels, const float* colors,
+                                 float min_weight, const std::int32_t* slots) {
   auto* mesh = new kfo_mesh();
   Volume vol = make_volume(g, kind, voxels, colors);
   if (vol.nx < 2 || vol.ny < 2 || vol.nz < 2) return mesh;
   const Sampler smp(&vol);
   std::unordered_map<std::size_t, std::uint32_t> welded;
   std::vector<V3> pos, nrm, col;
-  for (std::int64_t cz = 0; cz < vol.nz - 1; ++cz)
-    for (std::int64_t cy = 0; cy < vol.ny - 1; ++cy)
-      for (std::int64_t cx = 0; cx < vol.nx - 1; ++cx) {
+  auto add_cell = [&](std::int64_t cx, std::int64_t cy, std::int64_t cz) {
         float val[8];
         unsigned cube = 0;
-        bool ok = true;
         for (int c = 0; c < 8; ++c) {  // add_cell (:367-387)
           const Loaded v = vol.load(vol.flat(cx + kCorner[c][0], cy + kCorner[c][1], cz + kCorner[c][2]));
-          if (!v.observed || !v.finite || v.weight < min_weight) { ok = false; break; }
+          if (!v.observed || !v.finite || v.weight < min_weight) return;
           val[c] = v.tsdf;
           if (val[c] < 0.0F) cube |= 1U << c;
         }
-        if (!ok || cube == 0 || cube == 255) continue;
+        if (cube == 0 || cube == 255) return;
         const char* tri = kTriCases[cube];
         auto vertex_on_edge = [&](int edge) -> std::uint32_t {  // :412-448
           const int ne = kEdge[edge][0], fe = kEdge[edge][1];
@@ -1138,7 +1280,22 @@ void* kfo_mesh_extract(const kfo_geometry* g, int kind, const void* voxels, cons
           const std::uint32_t c = vertex_on_edge(edge_of(tri[e + 1]));
           mesh->triangles.insert(mesh->triangles.end(), {a, b, c});
         }
-      }
+  };
+  if (slots == nullptr) {
+    for (std::int64_t cz = 0; cz < vol.nz - 1; ++cz)
+      for (std::int64_t cy = 0; cy < vol.ny - 1; ++cy)
+        for (std::int64_t cx = 0; cx < vol.nx - 1; ++cx) add_cell(cx, cy, cz);
+  } else {
+    const BlockDims blocks = block_dims(vol);
+    for (std::int64_t b = 0; b < blocks.count(); ++b) {
+      if (slots[b] < 0) continue;
+      const std::int64_t bx = b % blocks.bx, by = (b / blocks.bx) % blocks.by, bz = b / (blocks.bx * blocks.by);
+      for (std::int64_t cz = bz * kBlockEdge; cz < cmin(vol.nz - 1, (bz + 1) * kBlockEdge); ++cz)
+        for (std::int64_t cy = by * kBlockEdge; cy < cmin(vol.ny - 1, (by + 1) * kBlockEdge); ++cy)
+          for (std::int64_t cx = bx * kBlockEdge; cx < cmin(vol.nx - 1, (bx + 1) * kBlockEdge); ++cx)
+            add_cell(cx, cy, cz);
+    }
+  }
   // fill_missing_normals (:452-467)
   std::unordered_map<std::uint32_t, V3> acc;
   const std::size_t ntri = mesh->triangles.size() / 3;
@@ -1160,6 +1317,11 @@ void* kfo_mesh_extract(const kfo_geometry* g, int kind, const void* voxels, cons
     mesh->colors.insert(mesh->colors.end(), {col[i].x, col[i].y, col[i].z});
   }
   return mesh;
+}
+
+void* kfo_mesh_extract(const kfo_geometry* g, int kind, const void* voxels, const float* colors,
+                       float min_weight) {
+  return kfo_mesh_extract_traversal(g, kind, voxels, colors, min_weight, nullptr);
 }
 
 void kfo_mesh_sizes(const void* m, std::int64_t* nverts, std::int64_t* ntris) {

@@ -117,6 +117,13 @@ def _declare(L: C.CDLL) -> None:
                                       C.POINTER(IcpOptions), C.POINTER(IcpResult)]
     L.kfo_mesh_extract.argtypes = [C.POINTER(Geometry), i32, vp, vp, f32]
     L.kfo_mesh_extract.restype = vp
+    L.kfo_mesh_extract_traversal.argtypes = [C.POINTER(Geometry), i32, vp, vp, f32, vp]
+    L.kfo_mesh_extract_traversal.restype = vp
+    L.kfo_band_walk.argtypes = [C.POINTER(Geometry), vp, i32, i32, C.POINTER(Intrinsics), vp,
+                                C.POINTER(IntegrationOptions), f32, vp, i64]
+    L.kfo_band_walk.restype = i64
+    L.kfo_integrate_sparse.argtypes = [C.POINTER(Geometry), i32, i32, vp, vp, vp, vp, vp, i32, i32, vp, vp,
+                                       C.POINTER(Intrinsics), vp, C.POINTER(IntegrationOptions), vp]
     L.kfo_mesh_sizes.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.kfo_mesh_copy.argtypes = [vp, vp, vp, vp, vp]
     L.kfo_mesh_free.argtypes = [vp]
@@ -195,6 +202,57 @@ def integrate(vol: Volume, rule: int, depth: np.ndarray, intr, world_to_camera, 
     lib().kfo_integrate(C.byref(vol.geometry), vol.kind, rule, _p(vol.voxels), _p(vol.colors), _p(depth), w, h,
                         _p(color), _p(normals), C.byref(k), _p(xf), C.byref(opts), _p(counts))
     return int(counts[0]), int(counts[1])
+
+
+class SparseVolume(Volume):
+    """BlockRep<kHost> (block_rep.hpp:183-307) kept as its dense expansion
+    plus the slot grid (slots[block] = pool slot or -1) and the allocation
+    state [requested, capacity]."""
+    slots: np.ndarray
+    state: np.ndarray
+
+    @staticmethod
+    def create_sparse(res, voxel_size, origin, truncation, kind=FLOAT, color=True, capacity=None) -> "SparseVolume":
+        base = Volume.create(res, voxel_size, origin, truncation, kind, color)
+        g = base.geometry
+        nb = ((g.nx + 7) // 8) * ((g.ny + 7) // 8) * ((g.nz + 7) // 8)
+        v = SparseVolume(base.geometry, base.kind, base.voxels, base.colors)
+        v.slots = np.full(nb, -1, np.int32)
+        v.state = np.array([0, capacity if capacity is not None else nb // 4], np.int64)
+        return v
+
+
+def integrate_sparse(vol: SparseVolume, rule: int, depth: np.ndarray, intr, world_to_camera, opts=None,
+                     color: np.ndarray | None = None, normals: np.ndarray | None = None) -> tuple[int, int, int]:
+    """BlockRep::integrate: (voxels fused, colour voxels fused, blocks listed)."""
+    h, w = depth.shape
+    depth = np.ascontiguousarray(depth, np.uint16)
+    if color is not None:
+        color = np.ascontiguousarray(color, np.uint32)
+    if normals is not None:
+        normals = np.ascontiguousarray(normals[..., :3], np.float32)
+    opts = opts or integration_options()
+    k = Intrinsics(*intr)
+    xf = pose12(world_to_camera)
+    counts = np.zeros(3, np.int64)
+    lib().kfo_integrate_sparse(C.byref(vol.geometry), vol.kind, rule, _p(vol.voxels), _p(vol.colors),
+                               _p(vol.slots), _p(vol.state), _p(depth), w, h, _p(color), _p(normals), C.byref(k),
+                               _p(xf), C.byref(opts), _p(counts))
+    return int(counts[0]), int(counts[1]), int(counts[2])
+
+
+def band_walk(geometry: Geometry, depth: np.ndarray, intr, world_to_camera, opts=None, step=None) -> np.ndarray:
+    """TruncationBandWalk::visit over all pixels: visited block indices in order."""
+    h, w = depth.shape
+    depth = np.ascontiguousarray(depth, np.uint16)
+    opts = opts or integration_options()
+    step = step if step is not None else 0.5 * 8 * geometry.voxel_size
+    k = Intrinsics(*intr)
+    xf = pose12(world_to_camera)
+    n = lib().kfo_band_walk(C.byref(geometry), _p(depth), w, h, C.byref(k), _p(xf), C.byref(opts), step, None, 0)
+    out = np.empty(max(n, 1), np.int64)
+    lib().kfo_band_walk(C.byref(geometry), _p(depth), w, h, C.byref(k), _p(xf), C.byref(opts), step, _p(out), n)
+    return out[:n]
 
 
 def observed_count(vol: Volume) -> int:
@@ -309,9 +367,12 @@ def icp_track_level(live_v, live_n, model_v, model_n, intr, model_pose, initial_
 
 
 def mesh(vol: Volume, min_weight=0.0):
-    """MarchingCubes::extract -> (positions, normals, colors, triangles)."""
+    """MarchingCubes::extract -> (positions, normals, colors, triangles);
+    the sparse traversal for a SparseVolume."""
     L = lib()
-    h = L.kfo_mesh_extract(C.byref(vol.geometry), vol.kind, _p(vol.voxels), _p(vol.colors), min_weight)
+    slots = vol.slots if isinstance(vol, SparseVolume) else None
+    h = L.kfo_mesh_extract_traversal(C.byref(vol.geometry), vol.kind, _p(vol.voxels), _p(vol.colors), min_weight,
+                                     _p(slots))
     nv, nt = C.c_int64(), C.c_int64()
     L.kfo_mesh_sizes(h, C.byref(nv), C.byref(nt))
     pos = np.empty((nv.value, 3), np.float32)

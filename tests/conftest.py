@@ -29,3 +29,11 @@ def torch_gpu():
     if not torch.cuda.is_available():
         pytest.fail("GPU test collected without a GPU: run with -m 'not gpu' on CPU hosts")
     return torch
+
+
+@pytest.fixture(scope="session")
+def scene_room():
+    """The seeded synthetic room (host renderer; no GPU needed)."""
+    import kinectfusion_amd
+
+    return kinectfusion_amd.SyntheticScene("room", seed=0)

@@ -1,0 +1,174 @@
+"""The BF16 MFMA pose network and the learned warm start against the FP64
+oracle (oracle/posenet_oracle.py) and against tracking without it.
+
+Bars: the forward pass and every layer's gradient within a stated relative
+error of the oracle evaluated on the same bf16-rounded operands (the
+difference is FP32 accumulation order and the rare bf16 rounding flip it
+causes); one Adam step within 1e-6 of the FP64 Adam update; training
+deterministic; the device renderer equal to the host renderer.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from kf_helpers import INTR, pipeline_config, to_dev
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 2e-2     # relative L2 error of the network output
+GRAD_TOL = 5e-2    # relative L2 error of each parameter tensor's gradient
+
+
+@pytest.fixture(scope="module")
+def K():
+    import kinectfusion_amd
+
+    return kinectfusion_amd
+
+
+@pytest.fixture(scope="module")
+def PN(K):
+    from kinectfusion_amd import posenet
+
+    return posenet
+
+
+@pytest.fixture(scope="module")
+def P():
+    from oracle import posenet_oracle
+
+    return posenet_oracle
+
+
+def random_pairs(torch, batch, seed=0):
+    rng = np.random.default_rng(seed)
+    raw = rng.integers(400, 3000, (batch, 2, 120, 160)).astype(np.uint16)
+    raw[rng.uniform(size=raw.shape) < 0.05] = 0
+    dev = torch.from_numpy(raw.view(np.int16)).cuda()
+    targets = rng.normal(0.0, 0.03, (batch, 6)).astype(np.float32)
+    return raw, dev, targets, torch.from_numpy(targets).cuda()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_oracle(PN, P, torch_gpu):
+    net = PN.PoseNet(PN.PoseNetConfig(batch_capacity=8, seed=3))
+    raw, dev, _, _ = random_pairs(torch_gpu, 4)
+    out = net.forward(dev).cpu().numpy()
+    ref, _ = P.forward(net.parameters().astype(np.float64), P.inputs_from_pairs(raw), emulate_bf16=True)
+    err = rel(out, ref * P.OUT_SCALE)
+    print(f"forward relative error {err:.2e}")
+    assert err < FWD_TOL
+    net.close()
+
+
+def test_gradients_match_oracle(PN, P, torch_gpu):
+    net = PN.PoseNet(PN.PoseNetConfig(batch_capacity=8, seed=5))
+    raw, dev, targets, tdev = random_pairs(torch_gpu, 8, seed=1)
+    loss, grads = net.gradients(dev, tdev)
+    ref_loss, ref = P.loss_and_grad(net.parameters().astype(np.float64), P.inputs_from_pairs(raw), targets,
+                                    emulate_bf16=True)
+    assert abs(loss - ref_loss) <= 1e-3 * ref_loss
+    off = 0
+    for l, (kind, cin, cout, hin, win, hout, wout, fan) in enumerate(P.layer_shapes()):
+        w_err = rel(grads[off: off + cout * fan], ref[off: off + cout * fan])
+        off += cout * fan
+        b_err = rel(grads[off: off + cout], ref[off: off + cout])
+        off += cout
+        print(f"layer {l}: weight gradient {w_err:.2e}, bias gradient {b_err:.2e}")
+        assert w_err < GRAD_TOL and b_err < GRAD_TOL
+    net.close()
+
+
+def test_adam_step_matches_oracle(PN, P, torch_gpu):
+    cfg = PN.PoseNetConfig(batch_capacity=8, seed=7, learning_rate=1e-3)
+    net = PN.PoseNet(cfg)
+    _, dev, _, tdev = random_pairs(torch_gpu, 8, seed=2)
+    p0 = net.parameters().astype(np.float64)
+    _, g = net.gradients(dev, tdev)
+    net.train_step(dev, tdev, report=True)
+    p1 = net.parameters()
+    ref, _, _ = P.adam(p0, g.astype(np.float64), np.zeros_like(p0), np.zeros_like(p0), 1, lr=cfg.learning_rate,
+                       beta1=cfg.beta1, beta2=cfg.beta2, eps=cfg.epsilon)
+    assert np.max(np.abs(p1 - ref)) <= 1e-6
+    net.close()
+
+
+def test_training_is_deterministic(PN, torch_gpu):
+    outs = []
+    for _ in range(2):
+        net = PN.PoseNet(PN.PoseNetConfig(batch_capacity=16, seed=11))
+        for s in range(3):
+            _, dev, _, tdev = random_pairs(torch_gpu, 16, seed=10 + s)
+            net.train_step(dev, tdev)
+        net.synchronize()
+        outs.append(net.parameters())
+        net.close()
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_device_renderer_matches_host(K, torch_gpu):
+    scene = K.SyntheticScene("room", seed=4)
+    poses = np.stack([scene.pose(i) for i in (0, 5, 9)])
+    keys = np.array([0, 5, 9], np.uint64)
+    dev = scene.render_device(poses, keys).cpu().numpy().view(np.uint16)
+    for j, i in enumerate((0, 5, 9)):
+        host, _, _ = scene.frame(i)
+        diff = dev[j].astype(np.int64) - host.astype(np.int64)
+        # the device's exp/log/cos may differ from glibc's in the last bit,
+        # which can move the noisy depth across a millimetre boundary
+        assert np.count_nonzero(diff) <= 1e-3 * diff.size and np.abs(diff).max() <= 1
+
+
+def test_network_learns_motion(K, PN, torch_gpu):
+    cfg = pipeline_config(K)
+    pairs, targets, stats = PN.generate_pairs(cfg, 2048, labels="truth", seed=1)
+    test_pairs, test_targets, _ = PN.generate_pairs(cfg, 256, labels="truth", seed=101, scene_seeds=(9,))
+    net = PN.PoseNet(PN.PoseNetConfig(seed=1))
+    before = PN.proposal_errors(net, test_pairs, test_targets)
+    hist = PN.train(net, pairs, targets, epochs=10, seed=1)
+    after = PN.proposal_errors(net, test_pairs, test_targets)
+    print(f"loss {hist}; held-out proposal error before {before}, after {after}")
+    assert hist[-1] < 0.5 * hist[0]
+    assert after[0] < 0.7 * before[0] and after[1] < 0.7 * before[1]
+    net.close()
+
+
+def test_learned_warm_start_tracks_and_gains(K, PN, torch_gpu):
+    """Train on rooms 1-8, track room 0 every fourth frame (up to ~4 cm and
+    2 deg between frames) with only 4 level-0 ICP iterations from the
+    proposal.  The network's gain over tracking without it: its proposal
+    lies closer to the tracked pose than the previous pose does (the
+    previous-pose seed of the reference), and the reduced schedule keeps
+    the trajectory within 1 cm of the ground truth."""
+    cfg = pipeline_config(K, voxel=0.02, init="learned", learned_iterations=(4, 0, 0))
+    pairs, targets, stats = PN.generate_pairs(cfg, 8192, seed=2)
+    print(f"training pairs: {stats}")
+    net = PN.PoseNet(PN.PoseNetConfig(seed=2))
+    hist = PN.train(net, pairs, targets, epochs=12, seed=2)
+    print(f"loss per epoch: {hist}")
+    scene = K.SyntheticScene("room", seed=0)
+    stride, n = 4, 12
+    p = K.Pipeline(cfg)
+    p.attach_posenet(net)
+    prop_err, used = [], 0
+    for i in range(n):
+        depth, rgba, _ = scene.frame(i * stride)
+        res = p.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba))
+        if i > 0:
+            assert res.tracked == 1
+            used += res.used_learned
+            prop_err.append(res.proposal_translation_error)
+    traj = p.trajectory()
+    motion = [np.linalg.norm((np.linalg.inv(traj[i - 1]) @ traj[i])[:3, 3]) for i in range(1, n)]
+    gt_err = [np.linalg.norm(traj[i][:3, 3] - scene.pose(i * stride)[:3, 3]) for i in range(n)]
+    print(f"used learned {used}/{n - 1}; proposal error median {np.median(prop_err) * 1000:.2f} mm; "
+          f"previous-pose error median {np.median(motion) * 1000:.2f} mm; max error vs truth {max(gt_err) * 1000:.2f} mm")
+    assert used >= (n - 1) * 0.8
+    assert np.median(prop_err) < np.median(motion)
+    assert max(gt_err) < 0.01
+    p.close()
+    net.close()
