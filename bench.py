@@ -1,17 +1,24 @@
 """Frames/s of the MI355X KinectFusion pipeline on the 512^3 precision
-workload, with the TSDF integration kernel's HBM roofline and a CPU baseline.
+workload: BF16 voxels with the learned ICP warm start against the plain FP32
+pipeline, with the TSDF integration kernel's HBM roofline and a CPU baseline.
 
-Workload ("512^3-bf16-100f"): 640x480 seeded synthetic room trajectory
+Workload ("512^3-5mm-room-640x480"): 640x480 seeded synthetic room trajectory
 (uint16 mm depth, sigma(z) = 0.0012 + 0.0019 (z - 0.4)^2 m, 1 % dropouts,
 RGBA8), fx = fy = 525, cx = 319.5, cy = 239.5; 512^3 voxels of 5 mm
 (2.56 m, camera margin 0.5 m), truncation 0.05 m, angle-weighted TSDF with
-colour, bilateral r = 3 / 4.5 px / 30 mm, 3-level ICP with 10/5/4
-iterations.  A step is one frame: pyramid + 3 tracking raycasts + ICP +
-integration.  Frames are resident in HBM before the timed region.
+colour, bilateral r = 3 / 4.5 px / 30 mm.  A step is one frame: pyramid,
+tracking (raycast + ICP per level), integration.  Frames are resident in HBM
+before the timed region.
 
-The headline line reports the BF16-voxel pipeline; the plain FP32 pipeline
-runs the same frames and is reported beside it (fp32_plain_fps,
-speedup_vs_fp32), with both trajectories' ATE against ground truth.
+Pipelines timed on the same frames:
+  * headline: BF16 voxels + learned warm start (pose network proposal, 4
+    level-0 ICP iterations, fallback to the constant-velocity seed with the
+    full 10/5/4 schedule when the final RMS exceeds 1 cm);
+  * plain FP32: float voxels, previous-pose seed, 10/5/4 iterations
+    (the reference's method);
+  * BF16 plain: bf16 voxels with the plain method.
+The pose network is trained first (untimed) on pairs the pipeline generates
+from rooms other than the evaluated one.
 """
 from __future__ import annotations
 
@@ -32,6 +39,7 @@ sys.path.insert(0, str(ROOT))
 INTR = (525.0, 525.0, 319.5, 239.5)
 RES, VOXEL, MARGIN, TRUNC = 512, 0.005, 0.5, 0.05
 HBM_PEAK_GBPS = 8000.0
+METRIC = "frames/s @512³ (bf16+learned-init vs FP32 plain, ATE≤5mm); TSDF HBM GB/s"
 
 
 def log(msg: str) -> None:
@@ -46,7 +54,8 @@ def make_config(K, voxel: str, init: str, **kw):
         integration=K.TsdfIntegrationOptions(depth_scale=1000.0),
         depth=K.DepthProcessingOptions(levels=3, bilateral_radius=3, bilateral_spatial_sigma=4.5,
                                        bilateral_depth_sigma=0.03),
-        intrinsics=K.CameraIntrinsics(*INTR), width=640, height=480, icp_iterations=(10, 5, 4), **kw)
+        intrinsics=K.CameraIntrinsics(*INTR), width=640, height=480, icp_iterations=(10, 5, 4),
+        learned_iterations=(4, 0, 0), learned_max_rms=0.01, **kw)
 
 
 def ate(traj: np.ndarray, gt: np.ndarray) -> float:
@@ -55,8 +64,10 @@ def ate(traj: np.ndarray, gt: np.ndarray) -> float:
     return ate_stats(traj, gt)["rmse"]
 
 
-def run_pipeline(K, torch, cfg, depth_dev, color_dev, warmup: int, steps: int):
+def run_pipeline(K, torch, cfg, depth_dev, color_dev, warmup: int, steps: int, net=None):
     p = K.Pipeline(cfg)
+    if net is not None:
+        p.attach_posenet(net)
     for i in range(warmup):
         p.process_frame(depth_dev[i], color_dev[i], report=False, sync_inputs=False)
     p.synchronize()
@@ -72,25 +83,57 @@ def run_pipeline(K, torch, cfg, depth_dev, color_dev, warmup: int, steps: int):
     return dt, traj
 
 
-def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int):
-    """Untimed replay with per-stage HIP events and exact fused-voxel counts."""
-    cfg.stage_timing = True
-    cfg.count_fused = True
+def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None):
+    """Untimed eager replays of the same frames: one with per-stage HIP
+    events on the library stream (the voxel-update kernel as it runs in the
+    timed loop), one counting the fused voxels of every frame (the kernel's
+    counting variant; identical poses and volumes, so identical counts)."""
+    cfg.stage_timing, cfg.count_fused = True, False
     p = K.Pipeline(cfg)
-    times = []
-    fused_before = (0, 0)
-    fused = []
+    if net is not None:
+        p.attach_posenet(net)
+    times, learned = [], 0
     for i in range(first + count):
-        p.process_frame(depth_dev[i], color_dev[i], report=True, sync_inputs=False)
+        res = p.process_frame(depth_dev[i], color_dev[i], report=True, sync_inputs=False)
         st = p.stage_times()
+        if i >= first:
+            times.append([st.pyramid_ms, st.proposal_ms, st.raycast_ms, st.icp_ms, st.integrate_ms,
+                          st.integrate_kernel_ms, st.total_ms])
+            learned += res.used_learned
+    p.close()
+    cfg.stage_timing, cfg.count_fused = False, True
+    p = K.Pipeline(cfg)
+    if net is not None:
+        p.attach_posenet(net)
+    fused, before = [], (0, 0)
+    for i in range(first + count):
+        p.process_frame(depth_dev[i], color_dev[i], report=False, sync_inputs=False)
         now = p.fused_counts()
         if i >= first:
-            times.append([st.pyramid_ms, st.raycast_ms, st.icp_ms, st.integrate_ms, st.integrate_kernel_ms,
-                          st.total_ms])
-            fused.append((now[0] - fused_before[0], now[1] - fused_before[1]))
-        fused_before = now
+            fused.append((now[0] - before[0], now[1] - before[1]))
+        before = now
     p.close()
-    return np.array(times), np.array(fused, np.float64)
+    cfg.count_fused = False
+    return np.array(times), np.array(fused, np.float64), learned
+
+
+def train_posenet(K, cfg, pairs_n: int, epochs: int):
+    from kinectfusion_amd import posenet as PN
+
+    t0 = time.perf_counter()
+    pairs, targets, stats = PN.generate_pairs(cfg, pairs_n, seed=7)
+    t1 = time.perf_counter()
+    net = PN.PoseNet(PN.PoseNetConfig(seed=7))
+    hist = PN.train(net, pairs, targets, epochs=epochs, seed=7, log=log)
+    t2 = time.perf_counter()
+    held, held_t, _ = PN.generate_pairs(cfg, 512, seed=99, scene_seeds=(11,))
+    err = PN.proposal_errors(net, held, held_t)
+    info = {"pairs": pairs_n, "epochs": epochs, "labels": "converged ICP", "icp_label_failures": stats["icp_failed"],
+            "generate_s": round(t1 - t0, 1), "train_s": round(t2 - t1, 1), "final_loss": round(hist[-1], 5),
+            "heldout_median_err_m": round(err[0], 5), "heldout_median_err_rad": round(err[1], 5)}
+    log(f"posenet: {info}")
+    del pairs, targets, held, held_t
+    return net, info
 
 
 def cpu_baseline(scene, frames: int):
@@ -151,7 +194,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cpu-frames", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["bf16", "float"], default=None, help="run one pipeline only (profiling)")
+    ap.add_argument("--train-pairs", type=int, default=20000)
+    ap.add_argument("--epochs", type=int, default=12)
+    ap.add_argument("--only", choices=["headline", "fp32"], default=None, help="time one pipeline only (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X; --gpus {args.gpus} is not "
@@ -176,26 +221,34 @@ def main() -> int:
     color_dev = [torch.from_numpy(c.view(np.int32)).cuda() for c in color_h]
     torch.cuda.synchronize()
 
+    net, net_info = None, None
+    if args.only in (None, "headline"):
+        log(f"training the pose network on {args.train_pairs} generated pairs, {args.epochs} epochs")
+        net, net_info = train_posenet(K, make_config(K, "bf16", "learned"), args.train_pairs, args.epochs)
+
+    runs = {"headline": ("bf16", "learned"), "fp32": ("float", "previous"), "bf16_plain": ("bf16", "previous")}
     results = {}
-    for name, voxel in (("bf16", "bf16"), ("float", "float")):
+    for name, (voxel, init) in runs.items():
         if args.only and args.only != name:
             continue
-        log(f"timing {name} pipeline: {args.warmup} warmup + {args.steps} timed frames")
-        dt, traj = run_pipeline(K, torch, make_config(K, voxel, "previous"), depth_dev, color_dev, args.warmup,
-                                args.steps)
+        log(f"timing {name} pipeline ({voxel} voxels, {init} seed): {args.warmup} warmup + {args.steps} frames")
+        dt, traj = run_pipeline(K, torch, make_config(K, voxel, init), depth_dev, color_dev, args.warmup, args.steps,
+                                net if init == "learned" else None)
         results[name] = {"seconds": dt, "fps": args.steps / dt, "ate_m": ate(traj, gt[: len(traj)])}
-        log(f"{name}: {args.steps / dt:.1f} frames/s, ATE {results[name]['ate_m'] * 1000:.2f} mm")
+        log(f"{name}: {args.steps / dt:.1f} frames/s, ATE {results[name]['ate_m'] * 1000:.3f} mm")
 
-    head = "bf16" if "bf16" in results else "float"
-    log("stage pass with HIP events and exact fused-voxel counts")
-    times, fused = stage_pass(K, make_config(K, head, "previous"), depth_dev, color_dev, args.warmup,
-                              min(args.steps, 40))
-    sv = 4 if head == "bf16" else 8
-    alg_bytes = fused[:, 0] * 2 * sv + fused[:, 1] * 2 * 16  # read + write of each fused voxel
-    kern_s = times[:, 4] / 1e3
-    achieved = float(alg_bytes.sum() / kern_s.sum() / 1e9)
-    stage_ms = {k: float(np.mean(times[:, j])) for j, k in
-                enumerate(["pyramid", "raycast", "icp", "integrate", "integrate_kernel", "total"])}
+    head = "headline" if "headline" in results else next(iter(results))
+    voxel, init = runs[head]
+    log("stage passes: HIP events on the library stream, then exact fused-voxel counts")
+    n_stage = min(args.steps, 40)
+    times, fused, learned_used = stage_pass(K, make_config(K, voxel, init), depth_dev, color_dev, args.warmup,
+                                            n_stage, net if init == "learned" else None)
+    sv = 4 if voxel == "bf16" else 8
+    alg_bytes = fused[:, 0] * 2 * sv + fused[:, 1] * 2 * 16  # read + write of each fused voxel (+ colour)
+    kern_ms = times[:, 5]
+    achieved = float(alg_bytes.sum() / (kern_ms.sum() / 1e3) / 1e9)
+    stage_ms = {k: round(float(np.mean(times[:, j])), 4) for j, k in
+                enumerate(["pyramid", "proposal", "raycast", "icp", "integrate", "integrate_kernel", "total"])}
 
     cpu = None
     if not args.no_cpu:
@@ -208,7 +261,7 @@ def main() -> int:
 
     r = results[head]
     line = {
-        "metric": "frames/s @512^3 (bf16 voxels; TSDF integration HBM GB/s in roofline)",
+        "metric": METRIC,
         "value": round(r["fps"], 2),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -218,24 +271,32 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16 voxels, fp32 arithmetic" if head == "bf16" else "fp32",
+        "dtype": "bf16 voxels, fp32 arithmetic, fp64 solve; bf16 MFMA pose network" if voxel == "bf16" else "fp32",
         "data": "synthetic",
-        "config": {"workload": "512^3-5mm-room-640x480", "frames": args.warmup + args.steps,
-                   "voxel": head, "init": "previous-pose", "icp_iterations": [10, 5, 4], "color": "float"},
+        "config": {"workload": "512^3-5mm-room-640x480", "frames": args.warmup + args.steps, "voxel": voxel,
+                   "init": init, "icp_iterations": [10, 5, 4],
+                   "learned_iterations": [4, 0, 0] if init == "learned" else None, "color": "float"},
         "ate_m": round(r["ate_m"], 6),
-        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stage_ms": stage_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                     "kernel": "integrate_kernel",
-                     "bytes_per_launch": float(alg_bytes.mean()),
-                     "avg_launch_ms": float(times[:, 4].mean())},
+                     "kernel": f"integrate_kernel<{voxel}, colour>",
+                     "bytes_per_launch": float(alg_bytes.mean()), "avg_launch_ms": float(kern_ms.mean())},
         "cpu_baseline": cpu,
     }
-    if "float" in results and head == "bf16":
-        line["fp32_plain_fps"] = round(results["float"]["fps"], 2)
-        line["fp32_plain_ate_m"] = round(results["float"]["ate_m"], 6)
-        line["speedup_vs_fp32"] = round(r["fps"] / results["float"]["fps"], 4)
+    if init == "learned":
+        line["learned_seed_kept"] = f"{learned_used}/{n_stage}"
+        line["posenet"] = net_info
+    if "fp32" in results:
+        line["fp32_plain_fps"] = round(results["fp32"]["fps"], 2)
+        line["fp32_plain_ate_m"] = round(results["fp32"]["ate_m"], 6)
+        line["speedup_vs_fp32"] = round(r["fps"] / results["fp32"]["fps"], 4)
+    if "bf16_plain" in results:
+        line["bf16_plain_fps"] = round(results["bf16_plain"]["fps"], 2)
+        line["bf16_plain_ate_m"] = round(results["bf16_plain"]["ate_m"], 6)
     print(json.dumps(line))
+    if net is not None:
+        net.close()
     return 0
 
 

@@ -575,6 +575,10 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
       ctx->obs_bits = dalloc<uint32_t>(ctx->block_words);
       ctx->skip_bits = dalloc<uint32_t>(ctx->block_words);
       KF_HIP(hipMemsetAsync(ctx->obs_bits, 0, ctx->block_words * 4, ctx->stream));
+      ctx->part_words = (static_cast<size_t>(bm.bx) * bm.by * bm.bz + 7) / 8;
+      ctx->part_bits = dalloc<uint32_t>(ctx->part_words);
+      ctx->free_bits = dalloc<uint32_t>(ctx->block_words);
+      KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
     }
     ctx->counters = dalloc<uint32_t>(4);
     ctx->fused = dalloc<unsigned long long>(4);
@@ -632,6 +636,8 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->bricks);
   dfree(ctx->obs_bits);
   dfree(ctx->skip_bits);
+  dfree(ctx->part_bits);
+  dfree(ctx->free_bits);
   dfree(ctx->counters);
   dfree(ctx->fused);
   dfree(ctx->partials);
@@ -665,6 +671,7 @@ kf_status kf_reset(kf_context* ctx) {
     launch_fill_voxels(ctx);
     if (hashed(ctx)) launch_fill_pool(ctx);
     KF_HIP(hipMemsetAsync(ctx->obs_bits, 0, ctx->block_words * 4, ctx->stream));
+    KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
     launch_skip_map(ctx);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
@@ -992,6 +999,8 @@ kf_status kf_volume_upload(kf_context* ctx, const void* voxels, const float* col
     if (colors != nullptr && ctx->colors != nullptr)
       KF_HIP(hipMemcpyAsync(ctx->colors, colors, ctx->n_voxels * sizeof(float4), hipMemcpyHostToDevice,
                             ctx->stream));
+    // uploaded voxels carry no free-space record until they are integrated again
+    KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
     launch_rebuild_observed(ctx);
     KF_HIP(hipStreamSynchronize(ctx->stream));
   });

@@ -238,33 +238,106 @@ __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
 }
 
 // 3. integrate the listed bricks.
+template <typename Raw>
+__device__ inline bool same_raw(Raw a, Raw b) {
+  if constexpr (sizeof(Raw) == 8) {
+    return __builtin_bit_cast(unsigned long long, a) == __builtin_bit_cast(unsigned long long, b);
+  } else {
+    return a == b;
+  }
+}
+
 // observe + fuse of one voxel (tsdf_integration.hpp:184-252); true when fused.
+//
+// Most voxels are decided without IEEE divisions: the projection and the
+// signed distance are first computed with the hardware reciprocal and square
+// root (relative error below 2^-21), and the exact IEEE computation runs only
+// when an approximate value lies within a safety margin of a decision
+// boundary (a half-integer pixel coordinate, -truncation, +truncation).  Away
+// from the boundaries the approximate value decides the same way as the exact
+// one, so the outcome is bit-identical:
+//  * the pixel: lround(u) = floor(u' + 0.5) when u' is farther than
+//    1e-3 + 2e-6 |q'| from a half-integer (|u'| < 4096, q' = fx x / z);
+//  * sd >= truncation (free space): tsdf = clamp(sd / trunc) = 1 exactly and
+//    no colour; the update ((t0 w0) + (1 w)) / (w0 + w) is 1 exactly when
+//    t0 = 1 (numerator and denominator are the same rounded sum), so the
+//    division runs only for t0 != 1;
+//  * sd < -truncation: rejected.
+// A voxel whose stored bits do not change (saturated free space) is not
+// written back.
 template <int Kind, bool Color, bool Count>
 __device__ inline bool fuse_voxel(const Grid& g, const FrameParams& f, const Rigid& w2c,
                                   const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
                                   typename VoxelIO<Kind>::Raw* __restrict__ vp, float4* __restrict__ cvp, int x,
-                                  int y, int z, unsigned int& my_fused, unsigned int& my_color) {
+                                  int y, int z, unsigned int& my_fused, unsigned int& my_color, bool& ones) {
   using IO = VoxelIO<Kind>;
+  ones = false;
   const V3 cp = xform(w2c, g.center(x, y, z));
   if (cp.z <= 0.0F) return false;
-  float u, v;
-  project(f.k, cp, u, v);
   int col, row;
-  if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) return false;
+  {
+    const float rz = __builtin_amdgcn_rcpf(cp.z);
+    const float qx = (f.k.fx * cp.x) * rz, qy = (f.k.fy * cp.y) * rz;
+    const float ua = qx + f.k.cx, va = qy + f.k.cy;
+    bool exact = !(fabsf(ua) < 4096.0F && fabsf(va) < 4096.0F);
+    if (!exact) {
+      const float eu = ua - floorf(ua), ev = va - floorf(va);
+      exact = fabsf(eu - 0.5F) <= 1.0e-3F + 2.0e-6F * fabsf(qx) || fabsf(ev - 0.5F) <= 1.0e-3F + 2.0e-6F * fabsf(qy);
+    }
+    if (exact) {
+      float u, v;
+      project(f.k, cp, u, v);
+      if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) return false;
+    } else {
+      col = static_cast<int>(floorf(ua + 0.5F));
+      row = static_cast<int>(floorf(va + 0.5F));
+      if (col < 0 || col >= f.w || row < 0 || row >= f.h) return false;
+    }
+  }
   const size_t pi = static_cast<size_t>(row) * f.w + col;
   const float4 p = pix[pi];
   if (p.x < 0.0F) return false;
-  const float sd = f.projective ? p.x - (norm3(cp) / p.y) : p.x - cp.z;
+  bool free_space;
+  float sd = 0.0F;
+  if (f.projective) {
+    const float sa = __builtin_amdgcn_sqrtf(dot3(cp, cp)) * __builtin_amdgcn_rcpf(p.y);
+    const float sda = p.x - sa;
+    const float margin = 1.0e-6F + 4.0e-6F * sa;
+    if (sda < -p.w - margin) return false;
+    free_space = sda >= p.w + margin;
+    if (!free_space) sd = p.x - (norm3(cp) / p.y);
+  } else {
+    sd = p.x - cp.z;
+    free_space = sd >= p.w;
+  }
+  const float w = p.z;
+  if (free_space) {
+    if (w <= 0.0F) return false;
+    const typename IO::Raw old = *vp;
+    float t0, w0;
+    bool obs, fin;
+    IO::unpack(old, t0, w0, obs, fin);
+    const float avg = t0 == 1.0F ? 1.0F : ((t0 * w0) + (1.0F * w)) / (w0 + w);
+    const float tw = fmin_ref(w0 + w, f.max_weight);
+    const typename IO::Raw nv = IO::pack(avg, tw);
+    if (!same_raw(nv, old)) *vp = nv;
+    IO::unpack(nv, t0, w0, obs, fin);
+    ones = t0 == 1.0F;
+    if (Count) ++my_fused;
+    return true;
+  }
   if (sd < -p.w) return false;
   const float tsdf = fclamp_ref(sd / p.w, -1.0F, 1.0F);
-  const float w = p.z;
   if (w <= 0.0F) return false;
   float t0, w0;
   bool obs, fin;
   IO::unpack(*vp, t0, w0, obs, fin);
   const float avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
   const float tw = fmin_ref(w0 + w, f.max_weight);
-  *vp = IO::pack(avg, tw);
+  const typename IO::Raw nv = IO::pack(avg, tw);
+  *vp = nv;
+  IO::unpack(nv, t0, w0, obs, fin);
+  ones = t0 == 1.0F;
   if (Count) ++my_fused;
   if (Color) {
     if (color_img != nullptr && sd <= p.w * 0.5F) {
@@ -288,16 +361,29 @@ __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, con
                                         const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
                                         float4* __restrict__ colors, int x, int y, int bz,
-                                        unsigned int& my_fused, unsigned int& my_color) {
+                                        unsigned int& my_fused, unsigned int& my_color, bool& all_ones) {
   bool any = false;
+  all_ones = true;
   for (int zz = 0; zz < kBrickZ; ++zz) {
     const int z = bz * kBrickZ + zz;
-    if (z >= g.nz) break;
+    if (z >= g.nz) {
+      all_ones = false;
+      break;
+    }
     const size_t vi = g.flat(x, y, z);
+    bool ones;
     any |= fuse_voxel<Kind, Color, Count>(g, f, w2c, pix, color_img, vox + vi, Color ? colors + vi : nullptr, x, y,
-                                          z, my_fused, my_color);
+                                          z, my_fused, my_color, ones);
+    all_ones = all_ones && ones;
   }
   return any;
+}
+
+// Part bits of block fb: bit `part` (y-half * 2 + z-half) of its nibble.
+__device__ inline void set_part(uint32_t* part_bits, int fb, int part, bool on) {
+  const uint32_t bit = 1U << ((fb & 7) * 4 + part);
+  if (on) atomicOr(&part_bits[fb >> 3], bit);
+  else atomicAnd(&part_bits[fb >> 3], ~bit);
 }
 
 template <int Kind, bool Color, bool Count>
@@ -309,7 +395,9 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
                                                          const uint32_t* __restrict__ list,
                                                          const uint32_t* __restrict__ count,
                                                          unsigned long long* __restrict__ fused,
-                                                         uint32_t* __restrict__ obs_bits, BlockMap bm) {
+                                                         uint32_t* __restrict__ obs_bits, BlockMap bm,
+                                                         uint32_t* __restrict__ part_bits) {
+  __shared__ uint32_t parts_ok;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
   const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
@@ -320,11 +408,19 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
     const uint32_t b = list[li];
     const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
     const int x = bx * kBrickX + tx, y = by * kBrickY + ty;
-    bool any = false;
+    bool any = false, ones = false;
+    if (threadIdx.x == 0) parts_ok = 0xFFU;
+    __syncthreads();
     // The brick's rows of 64 voxels span 8 occupancy blocks along x and one
     // along y and z: one ballot per wave marks them (lane 0 publishes).
     if (x < g.nx && y < g.ny) any = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors,
-                                                                           x, y, bz, my_fused, my_color);
+                                                                           x, y, bz, my_fused, my_color, ones);
+    const unsigned long long om = __ballot(ones);
+    if (tx == 0) {
+      uint32_t ok = 0U;
+      for (int j = 0; j < 8; ++j) ok |= ((om >> (8 * j)) & 0xFFULL) == 0xFFULL ? (1U << j) : 0U;
+      atomicAnd(&parts_ok, ok);
+    }
     const unsigned long long mask = __ballot(any);
     if (tx == 0 && mask != 0ULL) {
       const int blk_y = (by * kBrickY) >> 3, blk_z = (bz * kBrickZ) >> 3;
@@ -333,6 +429,17 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
         if (((mask >> (8 * j)) & 0xFFULL) == 0ULL || blk_x >= bm.bx) continue;
         const int fb = bm.flat(blk_x, blk_y, blk_z);
         atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      // this brick is the (y-half, z-half) part of 8 blocks along x
+      const int blk_y = (by * kBrickY) >> 3, blk_z = (bz * kBrickZ) >> 3;
+      const int part = (((by * kBrickY) >> 2) & 1) * 2 + (((bz * kBrickZ) >> 2) & 1);
+      for (int j = 0; j < 8; ++j) {
+        const int blk_x = bx * 8 + j;
+        if (blk_x >= bm.bx) break;
+        set_part(part_bits, bm.flat(blk_x, blk_y, blk_z), part, ((parts_ok >> j) & 1U) != 0U);
       }
     }
   }
@@ -361,10 +468,36 @@ __global__ void accumulate_counts_kernel(unsigned long long* f) {
 
 // skip bit of a block = the block and its 26 neighbours hold no observed
 // voxel.  One thread per block; a wave's 64 consecutive blocks form two words.
+__device__ inline bool block_free(const uint32_t* __restrict__ parts, int fb) {
+  return ((parts[fb >> 3] >> ((fb & 7) * 4)) & 0xFU) == 0xFU;
+}
+
+// free bit of a block = the block and its 26 neighbours (all inside the
+// grid) hold only fused voxels reading tsdf exactly 1.
 __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restrict__ obs, uint32_t* __restrict__ skip,
-                                                        BlockMap bm, int nblocks) {
+                                                        const uint32_t* __restrict__ parts,
+                                                        uint32_t* __restrict__ free_bits, BlockMap bm, int nblocks) {
   const int b = blockIdx.x * 256 + threadIdx.x;
-  bool empty = false;
+  bool empty = false, free_nb = false;
+  if (b < nblocks) {
+    const int x = b % bm.bx, y = (b / bm.bx) % bm.by, z = b / (bm.bx * bm.by);
+    free_nb = x > 0 && y > 0 && z > 0 && x + 1 < bm.bx && y + 1 < bm.by && z + 1 < bm.bz;
+    for (int dz = -1; dz <= 1 && free_nb; ++dz)
+      for (int dy = -1; dy <= 1 && free_nb; ++dy)
+        for (int dx = -1; dx <= 1; ++dx)
+          if (!block_free(parts, bm.flat(x + dx, y + dy, z + dz))) {
+            free_nb = false;
+            break;
+          }
+  }
+  {
+    const unsigned long long m = __ballot(free_nb);
+    const int lane = threadIdx.x & 63;
+    const int word0 = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 5;
+    const int nwords = (nblocks + 31) >> 5;
+    if (lane == 0 && word0 < nwords) free_bits[word0] = static_cast<uint32_t>(m);
+    if (lane == 32 && word0 + 1 < nwords) free_bits[word0 + 1] = static_cast<uint32_t>(m >> 32);
+  }
   if (b < nblocks) {
     const int x = b % bm.bx, y = (b / bm.bx) % bm.by, z = b / (bm.bx * bm.by);
     empty = true;
@@ -563,8 +696,10 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
                                                                 const uint32_t* __restrict__ work,
                                                                 const uint32_t* __restrict__ sp,
                                                                 unsigned long long* __restrict__ fused,
-                                                                uint32_t* __restrict__ obs_bits, BlockMap bm) {
+                                                                uint32_t* __restrict__ obs_bits, BlockMap bm,
+                                                                uint32_t* __restrict__ part_bits) {
   __shared__ int any_fused;
+  __shared__ int all_ones;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
   const uint32_t n = sp[1];
@@ -575,23 +710,32 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
     const int bx = static_cast<int>(key % static_cast<uint32_t>(bm.bx));
     const int by = static_cast<int>((key / static_cast<uint32_t>(bm.bx)) % static_cast<uint32_t>(bm.by));
     const int bz = static_cast<int>(key / static_cast<uint32_t>(bm.bx * bm.by));
-    if (threadIdx.x == 0) any_fused = 0;
+    if (threadIdx.x == 0) {
+      any_fused = 0;
+      all_ones = 1;
+    }
     __syncthreads();
-    bool any = false;
+    bool any = false, ones_all = true;
     for (int pass = 0; pass < 2; ++pass) {
       const int i = static_cast<int>(threadIdx.x) + pass * 256;
       const int x = bx * 8 + (i & 7), y = by * 8 + ((i >> 3) & 7), z = bz * 8 + (i >> 6);
+      bool ones = false;
       if (x < g.nx && y < g.ny && z < g.nz) {
         const size_t vi = static_cast<size_t>(slot) * kBlockVoxels + i;
         any |= fuse_voxel<Kind, Color, Count>(g, f, w2c, pix, color_img, pool + vi, Color ? cpool + vi : nullptr, x,
-                                              y, z, my_fused, my_color);
+                                              y, z, my_fused, my_color, ones);
       }
+      ones_all = ones_all && ones;
     }
     if (any) any_fused = 1;
+    if (!ones_all) all_ones = 0;
     __syncthreads();
-    if (threadIdx.x == 0 && any_fused != 0) {
+    if (threadIdx.x == 0) {
       const int fb = bm.flat(bx, by, bz);
-      atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
+      if (any_fused != 0) atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
+      const uint32_t nib = 0xFU << ((fb & 7) * 4);
+      if (all_ones != 0) atomicOr(&part_bits[fb >> 3], nib);
+      else atomicAnd(&part_bits[fb >> 3], ~nib);
     }
     __syncthreads();
   }
@@ -623,11 +767,13 @@ void launch_blocks_t(kf_context* ctx, const FrameParams& f, const uint32_t* colo
   if (count) {
     hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
-                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid));
+                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid),
+                       ctx->part_bits);
   } else {
     hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, false>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
-                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid));
+                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid),
+                       ctx->part_bits);
   }
 }
 
@@ -657,12 +803,12 @@ void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* c
     hipLaunchKernelGGL((integrate_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
                        ctx->colors, ctx->bricks, ctx->counters, ctx->fused, ctx->obs_bits,
-                       block_map(ctx->grid));
+                       block_map(ctx->grid), ctx->part_bits);
   } else {
     hipLaunchKernelGGL((integrate_kernel<Kind, Color, false>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
                        ctx->colors, ctx->bricks, ctx->counters, ctx->fused, ctx->obs_bits,
-                       block_map(ctx->grid));
+                       block_map(ctx->grid), ctx->part_bits);
   }
 }
 
@@ -725,7 +871,7 @@ void launch_skip_map(kf_context* ctx) {
   const BlockMap bm = block_map(ctx->grid);
   const int nblocks = bm.bx * bm.by * bm.bz;
   hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
-                     ctx->skip_bits, bm, nblocks);
+                     ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks);
   KF_HIP(hipGetLastError());
 }
 

@@ -334,6 +334,13 @@ struct kf_context {
   // neighbours all unobserved (the raycaster skips those)
   uint32_t* obs_bits = nullptr;
   uint32_t* skip_bits = nullptr;
+  // free space: 4 bits per block (its y-half x z-half parts) set when every
+  // voxel of the part was fused last and reads tsdf exactly 1; a block's free
+  // bit = all 4 parts set for it and its 26 neighbours (the raycaster steps
+  // through those blocks without loads)
+  uint32_t* part_bits = nullptr;
+  uint32_t* free_bits = nullptr;
+  size_t part_words = 0;
   size_t block_words = 0;
   uint32_t* counters = nullptr;   // [0] brick count, [1..2] fused counts (u64 pair as 2x u32), spare
   unsigned long long* fused = nullptr;

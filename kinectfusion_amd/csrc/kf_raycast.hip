@@ -43,7 +43,8 @@ template <int Kind, typename Store>
 __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, RayParams rp, Rigid pose_arg,
                                                        IcpState* state, float4* __restrict__ pts,
                                                        float4* __restrict__ nrm, uint32_t* __restrict__ col,
-                                                       const uint32_t* __restrict__ skip_bits, BlockMap bm) {
+                                                       const uint32_t* __restrict__ skip_bits,
+                                                       const uint32_t* __restrict__ free_bits, BlockMap bm) {
   Rigid pose = pose_arg;
   if (state != nullptr) {
     if (state->frame_failed) return;
@@ -64,11 +65,14 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
     float prev_t = 0.0F;
     V3 prev_p = mk3(0.0F, 0.0F, 0.0F), surface = prev_p;
     float len = rp.min_depth;
-    // Samples that provably return no TSDF are stepped over without touching
-    // the volume, reproducing the reference's float sequence of ray lengths:
+    // Samples whose outcome is known are stepped over without touching the
+    // volume, reproducing the reference's float sequence of ray lengths:
     // before the ray enters the volume box, after it has left it (it cannot
-    // come back), and inside 8^3 blocks whose 3x3x3 block neighbourhood holds
-    // no observed voxel.  The output is identical to marching every sample.
+    // come back), inside 8^3 blocks whose 3x3x3 block neighbourhood holds no
+    // observed voxel (no TSDF), and inside blocks whose neighbourhood holds
+    // only observed voxels reading exactly 1 (the trilinear blend of eight
+    // corners of 1 is exactly 1: the weighted sum and the weight sum are the
+    // same float sum).  The output is identical to marching every sample.
     const double inv = 1.0 / static_cast<double>(vol.g.voxel);
     const double ga[3] = {(static_cast<double>(pose.t.x) - vol.g.ox) * inv,
                           (static_cast<double>(pose.t.y) - vol.g.oy) * inv,
@@ -110,13 +114,33 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
       const V3 gc = vol.grid_coords(point);
       if (vol.contains(gc)) {
         const int bx = static_cast<int>(gc.x) >> 3, by = static_cast<int>(gc.y) >> 3, bz = static_cast<int>(gc.z) >> 3;
-        if (bm.bit(skip_bits, bm.flat(bx, by, bz))) {
+        const int fb = bm.flat(bx, by, bz);
+        if (bm.bit(skip_bits, fb)) {
           const double lo[3] = {8.0 * bx, 8.0 * by, 8.0 * bz};
           const double hi[3] = {lo[0] + 8.0, lo[1] + 8.0, lo[2] + 8.0};
           const double t_blk = box_exit(ga, gb, lo, hi);
           have_prev = false;
           len += base_step;  // this sample, then every sample still in the block
           while (static_cast<double>(len) <= t_blk && len <= rp.max_depth) len += base_step;
+          continue;
+        }
+        if (bm.bit(free_bits, fb)) {
+          // every sample in the block reads t = 1: no crossing; a miss if the
+          // previous sample was negative; otherwise step by max(base, trunc)
+          if (have_prev && prev_t < 0.0F) break;
+          const double lo[3] = {8.0 * bx, 8.0 * by, 8.0 * bz};
+          const double hi[3] = {lo[0] + 8.0, lo[1] + 8.0, lo[2] + 8.0};
+          const double t_blk = box_exit(ga, gb, lo, hi);
+          const float free_step = fmax_ref(base_step, (1.0F * vol.g.trunc) * rp.step_scale);
+          float last = len;
+          len += free_step;
+          while (static_cast<double>(len) <= t_blk && len <= rp.max_depth && !(len > len_stop)) {
+            last = len;
+            len += free_step;
+          }
+          have_prev = true;
+          prev_t = 1.0F;
+          prev_p = add3(pose.t, scale3(dir, last));
           continue;
         }
       }
@@ -175,13 +199,13 @@ void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState*
         HashedStore<Kind>{hash_view(ctx), static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
         1.0F / ctx->grid.voxel};
     hipLaunchKernelGGL((raycast_kernel<Kind, HashedStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose,
-                       state, L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, block_map(ctx->grid));
+                       state, L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, block_map(ctx->grid));
     return;
   }
   Sampler<Kind> vol{DenseStore<Kind>{static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
                     1.0F / ctx->grid.voxel};
   hipLaunchKernelGGL((raycast_kernel<Kind, DenseStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,
-                     L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, block_map(ctx->grid));
+                     L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, block_map(ctx->grid));
 }
 
 }  // namespace
