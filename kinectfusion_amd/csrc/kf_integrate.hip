@@ -247,7 +247,11 @@ __device__ inline bool same_raw(Raw a, Raw b) {
   }
 }
 
-// observe + fuse of one voxel (tsdf_integration.hpp:184-252); true when fused.
+// observe + fuse (tsdf_integration.hpp:184-252) of N voxels of one thread,
+// in phases so that the N pixel-record loads, then the N voxel loads, are in
+// flight together: (1) project, (2) read the pixel records, (3) decide,
+// (4) read the voxels, (5) update.  any: some voxel fused; all_ones: every
+// voxel fused and reads tsdf exactly 1 afterwards (free-space part bits).
 //
 // Most voxels are decided without IEEE divisions: the projection and the
 // signed distance are first computed with the hardware reciprocal and square
@@ -265,117 +269,139 @@ __device__ inline bool same_raw(Raw a, Raw b) {
 //  * sd < -truncation: rejected.
 // A voxel whose stored bits do not change (saturated free space) is not
 // written back.
-template <int Kind, bool Color, bool Count>
-__device__ inline bool fuse_voxel(const Grid& g, const FrameParams& f, const Rigid& w2c,
+template <int Kind, bool Color, bool Count, int N>
+__device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rigid& w2c,
                                   const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
-                                  typename VoxelIO<Kind>::Raw* __restrict__ vp, float4* __restrict__ cvp, int x,
-                                  int y, int z, unsigned int& my_fused, unsigned int& my_color, bool& ones) {
+                                  typename VoxelIO<Kind>::Raw* __restrict__ vox, float4* __restrict__ colors,
+                                  const int (&xs)[N], const int (&ys)[N], const int (&zs)[N],
+                                  const size_t (&vis)[N], const bool (&inside)[N], unsigned int& my_fused,
+                                  unsigned int& my_color, bool& any, bool& all_ones) {
   using IO = VoxelIO<Kind>;
-  ones = false;
-  const V3 cp = xform(w2c, g.center(x, y, z));
-  if (cp.z <= 0.0F) return false;
-  int col, row;
-  {
-    const float rz = __builtin_amdgcn_rcpf(cp.z);
-    const float qx = (f.k.fx * cp.x) * rz, qy = (f.k.fy * cp.y) * rz;
+  int pi[N];
+  V3 cp[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    pi[k] = -1;
+    cp[k] = mk3(0.0F, 0.0F, 0.0F);
+    if (!inside[k]) continue;
+    cp[k] = xform(w2c, g.center(xs[k], ys[k], zs[k]));
+    if (cp[k].z <= 0.0F) continue;
+    const float rz = __builtin_amdgcn_rcpf(cp[k].z);
+    const float qx = (f.k.fx * cp[k].x) * rz, qy = (f.k.fy * cp[k].y) * rz;
     const float ua = qx + f.k.cx, va = qy + f.k.cy;
     bool exact = !(fabsf(ua) < 4096.0F && fabsf(va) < 4096.0F);
     if (!exact) {
       const float eu = ua - floorf(ua), ev = va - floorf(va);
       exact = fabsf(eu - 0.5F) <= 1.0e-3F + 2.0e-6F * fabsf(qx) || fabsf(ev - 0.5F) <= 1.0e-3F + 2.0e-6F * fabsf(qy);
     }
+    int col, row;
     if (exact) {
       float u, v;
-      project(f.k, cp, u, v);
-      if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) return false;
+      project(f.k, cp[k], u, v);
+      if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) continue;
     } else {
       col = static_cast<int>(floorf(ua + 0.5F));
       row = static_cast<int>(floorf(va + 0.5F));
-      if (col < 0 || col >= f.w || row < 0 || row >= f.h) return false;
+      if (col < 0 || col >= f.w || row < 0 || row >= f.h) continue;
     }
+    pi[k] = row * f.w + col;
   }
-  const size_t pi = static_cast<size_t>(row) * f.w + col;
-  const float4 p = pix[pi];
-  if (p.x < 0.0F) return false;
-  bool free_space;
-  float sd = 0.0F;
-  if (f.projective) {
-    const float sa = __builtin_amdgcn_sqrtf(dot3(cp, cp)) * __builtin_amdgcn_rcpf(p.y);
-    const float sda = p.x - sa;
-    const float margin = 1.0e-6F + 4.0e-6F * sa;
-    if (sda < -p.w - margin) return false;
-    free_space = sda >= p.w + margin;
-    if (!free_space) sd = p.x - (norm3(cp) / p.y);
-  } else {
-    sd = p.x - cp.z;
-    free_space = sd >= p.w;
+  float4 p[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) p[k] = pi[k] >= 0 ? pix[pi[k]] : make_float4(-1.0F, 1.0F, 0.0F, 1.0F);
+  // mode: 0 no fuse, 1 free space (tsdf 1, no colour), 2 in the band (exact sd)
+  int mode[N];
+  float sd[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    mode[k] = 0;
+    sd[k] = 0.0F;
+    if (p[k].x < 0.0F || p[k].z <= 0.0F) continue;  // no depth, or w <= 0 drops the fuse
+    if (f.projective) {
+      const float sa = __builtin_amdgcn_sqrtf(dot3(cp[k], cp[k])) * __builtin_amdgcn_rcpf(p[k].y);
+      const float sda = p[k].x - sa;
+      const float margin = 1.0e-6F + 4.0e-6F * sa;
+      if (sda < -p[k].w - margin) continue;
+      if (sda >= p[k].w + margin) {
+        mode[k] = 1;
+        continue;
+      }
+      sd[k] = p[k].x - (norm3(cp[k]) / p[k].y);
+    } else {
+      sd[k] = p[k].x - cp[k].z;
+      if (sd[k] >= p[k].w) {
+        mode[k] = 1;
+        continue;
+      }
+    }
+    if (sd[k] < -p[k].w) continue;
+    mode[k] = 2;
   }
-  const float w = p.z;
-  if (free_space) {
-    if (w <= 0.0F) return false;
-    const typename IO::Raw old = *vp;
+  typename IO::Raw raw[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) raw[k] = mode[k] != 0 ? vox[vis[k]] : IO::fresh();
+  all_ones = true;
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (mode[k] == 0) {
+      all_ones = false;
+      continue;
+    }
+    const float w = p[k].z;
     float t0, w0;
     bool obs, fin;
-    IO::unpack(old, t0, w0, obs, fin);
-    const float avg = t0 == 1.0F ? 1.0F : ((t0 * w0) + (1.0F * w)) / (w0 + w);
+    IO::unpack(raw[k], t0, w0, obs, fin);
+    float avg;
+    if (mode[k] == 1) {
+      avg = t0 == 1.0F ? 1.0F : ((t0 * w0) + (1.0F * w)) / (w0 + w);
+    } else {
+      const float tsdf = fclamp_ref(sd[k] / p[k].w, -1.0F, 1.0F);
+      avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
+    }
     const float tw = fmin_ref(w0 + w, f.max_weight);
     const typename IO::Raw nv = IO::pack(avg, tw);
-    if (!same_raw(nv, old)) *vp = nv;
+    if (!same_raw(nv, raw[k])) vox[vis[k]] = nv;
     IO::unpack(nv, t0, w0, obs, fin);
-    ones = t0 == 1.0F;
+    all_ones = all_ones && t0 == 1.0F;
+    any = true;
     if (Count) ++my_fused;
-    return true;
-  }
-  if (sd < -p.w) return false;
-  const float tsdf = fclamp_ref(sd / p.w, -1.0F, 1.0F);
-  if (w <= 0.0F) return false;
-  float t0, w0;
-  bool obs, fin;
-  IO::unpack(*vp, t0, w0, obs, fin);
-  const float avg = ((t0 * w0) + (tsdf * w)) / (w0 + w);
-  const float tw = fmin_ref(w0 + w, f.max_weight);
-  const typename IO::Raw nv = IO::pack(avg, tw);
-  *vp = nv;
-  IO::unpack(nv, t0, w0, obs, fin);
-  ones = t0 == 1.0F;
-  if (Count) ++my_fused;
-  if (Color) {
-    if (color_img != nullptr && sd <= p.w * 0.5F) {
-      const uint32_t px = color_img[pi];
-      const V3 c = mk3(static_cast<float>(px & 0xFFU), static_cast<float>((px >> 8U) & 0xFFU),
-                       static_cast<float>((px >> 16U) & 0xFFU));
-      const float4 old = *cvp;
-      const float cw = old.w;
-      const float den = cw + w;
-      *cvp = make_float4(((old.x * cw) + (c.x * w)) / den, ((old.y * cw) + (c.y * w)) / den,
-                         ((old.z * cw) + (c.z * w)) / den, fmin_ref(cw + w, f.max_weight));
-      if (Count) ++my_color;
+    if (Color && mode[k] == 2) {
+      if (color_img != nullptr && sd[k] <= p[k].w * 0.5F) {
+        const uint32_t px = color_img[pi[k]];
+        const V3 c = mk3(static_cast<float>(px & 0xFFU), static_cast<float>((px >> 8U) & 0xFFU),
+                         static_cast<float>((px >> 16U) & 0xFFU));
+        const float4 old = colors[vis[k]];
+        const float cw = old.w;
+        const float den = cw + w;
+        colors[vis[k]] = make_float4(((old.x * cw) + (c.x * w)) / den, ((old.y * cw) + (c.y * w)) / den,
+                                     ((old.z * cw) + (c.z * w)) / den, fmin_ref(cw + w, f.max_weight));
+        if (Count) ++my_color;
+      }
     }
   }
-  return true;
 }
 
-// The voxel updates of one (x, y) column of a brick; true when any fused.
+// The voxel updates of one (x, y) column of a brick.
 template <int Kind, bool Color, bool Count>
 __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, const Rigid& w2c,
                                         const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
                                         float4* __restrict__ colors, int x, int y, int bz,
                                         unsigned int& my_fused, unsigned int& my_color, bool& all_ones) {
-  bool any = false;
-  all_ones = true;
-  for (int zz = 0; zz < kBrickZ; ++zz) {
-    const int z = bz * kBrickZ + zz;
-    if (z >= g.nz) {
-      all_ones = false;
-      break;
-    }
-    const size_t vi = g.flat(x, y, z);
-    bool ones;
-    any |= fuse_voxel<Kind, Color, Count>(g, f, w2c, pix, color_img, vox + vi, Color ? colors + vi : nullptr, x, y,
-                                          z, my_fused, my_color, ones);
-    all_ones = all_ones && ones;
+  int xs[kBrickZ], ys[kBrickZ], zs[kBrickZ];
+  size_t vis[kBrickZ];
+  bool inside[kBrickZ];
+#pragma unroll
+  for (int k = 0; k < kBrickZ; ++k) {
+    xs[k] = x;
+    ys[k] = y;
+    zs[k] = bz * kBrickZ + k;
+    inside[k] = zs[k] < g.nz;
+    vis[k] = inside[k] ? g.flat(x, y, zs[k]) : 0;
   }
+  bool any = false;
+  fuse_batch<Kind, Color, Count, kBrickZ>(g, f, w2c, pix, color_img, vox, colors, xs, ys, zs, vis, inside, my_fused,
+                                          my_color, any, all_ones);
   return any;
 }
 
@@ -715,18 +741,21 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
       all_ones = 1;
     }
     __syncthreads();
-    bool any = false, ones_all = true;
+    int xs[2], ys[2], zs[2];
+    size_t vis[2];
+    bool inside[2];
+#pragma unroll
     for (int pass = 0; pass < 2; ++pass) {
       const int i = static_cast<int>(threadIdx.x) + pass * 256;
-      const int x = bx * 8 + (i & 7), y = by * 8 + ((i >> 3) & 7), z = bz * 8 + (i >> 6);
-      bool ones = false;
-      if (x < g.nx && y < g.ny && z < g.nz) {
-        const size_t vi = static_cast<size_t>(slot) * kBlockVoxels + i;
-        any |= fuse_voxel<Kind, Color, Count>(g, f, w2c, pix, color_img, pool + vi, Color ? cpool + vi : nullptr, x,
-                                              y, z, my_fused, my_color, ones);
-      }
-      ones_all = ones_all && ones;
+      xs[pass] = bx * 8 + (i & 7);
+      ys[pass] = by * 8 + ((i >> 3) & 7);
+      zs[pass] = bz * 8 + (i >> 6);
+      inside[pass] = xs[pass] < g.nx && ys[pass] < g.ny && zs[pass] < g.nz;
+      vis[pass] = static_cast<size_t>(slot) * kBlockVoxels + i;
     }
+    bool any = false, ones_all = true;
+    fuse_batch<Kind, Color, Count, 2>(g, f, w2c, pix, color_img, pool, cpool, xs, ys, zs, vis, inside, my_fused,
+                                      my_color, any, ones_all);
     if (any) any_fused = 1;
     if (!ones_all) all_ones = 0;
     __syncthreads();

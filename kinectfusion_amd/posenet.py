@@ -191,10 +191,11 @@ def random_view(rng: np.random.Generator) -> np.ndarray:
     return T
 
 
-def generate_pairs(config: PipelineConfig, count: int, scene_seeds=(1, 2, 3, 4, 5, 6, 7, 8), seed: int = 0,
+def generate_pairs(config: PipelineConfig, count: int, scene_seeds=tuple(range(1, 65)), seed: int = 0,
                    max_translation: float = 0.05, max_rotation_deg: float = 5.0, labels: str = "icp",
-                   chunk: int = 256):
-    """`count` (pairs, targets, stats) from seeded synthetic rooms.
+                   chunk: int = 128):
+    """`count` (pairs, targets, stats) from seeded synthetic rooms, a new
+    room (from scene_seeds, in turn) every `chunk` pairs.
 
     Each pair is a random view and the same view moved by up to
     max_translation / max_rotation_deg.  Both frames are rendered on the GPU,
