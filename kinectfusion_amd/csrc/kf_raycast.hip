@@ -108,39 +108,32 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
     const float len_stop = t_out < t_in ? -1.0F : static_cast<float>(fmin(t_out + 0.01 / slope_out, 1.0e30));
     const double len_enter = t_in - 0.01 / slope_in;
     while (static_cast<double>(len) < len_enter && len <= rp.max_depth) len += base_step;
+    const float free_step = fmax_ref(base_step, (1.0F * vol.g.trunc) * rp.step_scale);
+    int cur_blk = -1;
+    bool cur_skip = false, cur_free = false;
     while (len <= rp.max_depth) {
       if (len > len_stop) break;
       const V3 point = add3(pose.t, scale3(dir, len));
       const V3 gc = vol.grid_coords(point);
       if (vol.contains(gc)) {
-        const int bx = static_cast<int>(gc.x) >> 3, by = static_cast<int>(gc.y) >> 3, bz = static_cast<int>(gc.z) >> 3;
-        const int fb = bm.flat(bx, by, bz);
-        if (bm.bit(skip_bits, fb)) {
-          const double lo[3] = {8.0 * bx, 8.0 * by, 8.0 * bz};
-          const double hi[3] = {lo[0] + 8.0, lo[1] + 8.0, lo[2] + 8.0};
-          const double t_blk = box_exit(ga, gb, lo, hi);
+        // the block of this sample, classified exactly as the sampler sees it
+        const int fb = bm.flat(static_cast<int>(gc.x) >> 3, static_cast<int>(gc.y) >> 3, static_cast<int>(gc.z) >> 3);
+        if (fb != cur_blk) {
+          cur_blk = fb;
+          cur_skip = bm.bit(skip_bits, fb);
+          cur_free = !cur_skip && bm.bit(free_bits, fb);
+        }
+        if (cur_skip) {  // no corner is observed: no TSDF
           have_prev = false;
-          len += base_step;  // this sample, then every sample still in the block
-          while (static_cast<double>(len) <= t_blk && len <= rp.max_depth) len += base_step;
+          len += base_step;
           continue;
         }
-        if (bm.bit(free_bits, fb)) {
-          // every sample in the block reads t = 1: no crossing; a miss if the
-          // previous sample was negative; otherwise step by max(base, trunc)
+        if (cur_free) {  // every corner reads exactly 1
           if (have_prev && prev_t < 0.0F) break;
-          const double lo[3] = {8.0 * bx, 8.0 * by, 8.0 * bz};
-          const double hi[3] = {lo[0] + 8.0, lo[1] + 8.0, lo[2] + 8.0};
-          const double t_blk = box_exit(ga, gb, lo, hi);
-          const float free_step = fmax_ref(base_step, (1.0F * vol.g.trunc) * rp.step_scale);
-          float last = len;
-          len += free_step;
-          while (static_cast<double>(len) <= t_blk && len <= rp.max_depth && !(len > len_stop)) {
-            last = len;
-            len += free_step;
-          }
           have_prev = true;
           prev_t = 1.0F;
-          prev_p = add3(pose.t, scale3(dir, last));
+          prev_p = point;
+          len += free_step;
           continue;
         }
       }
