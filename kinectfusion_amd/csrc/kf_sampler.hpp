@@ -24,6 +24,7 @@ struct SlotCache {
 template <int Kind>
 struct DenseStore {
   using Raw = typename VoxelIO<Kind>::Raw;
+  static constexpr bool kDirect = true;  // a voxel is one load
   const Raw* __restrict__ vox;
   const float4* __restrict__ colors;
   __device__ inline Raw voxel(const Grid& g, int x, int y, int z) const { return vox[g.flat(x, y, z)]; }
@@ -40,6 +41,7 @@ struct DenseStore {
 template <int Kind>
 struct HashedStore {
   using Raw = typename VoxelIO<Kind>::Raw;
+  static constexpr bool kDirect = false;  // a voxel needs a hash probe first
   HashView hv;
   const Raw* __restrict__ pool;
   const float4* __restrict__ cpool;
@@ -86,6 +88,67 @@ struct Sampler {
     return scale3(sub3(p, mk3(g.ox, g.oy, g.oz)), inv_voxel);
   }
   __device__ static inline float axis_w(float f, int o) { return o == 0 ? 1.0F - f : f; }
+
+  // The 2^3 stencil of a point: base corner, fractions, in-volume flag.
+  struct Stencil {
+    int bx, by, bz;
+    V3 fr;
+    bool ok;        // the point lies inside the grid
+    bool interior;  // all eight corners lie inside the grid
+  };
+  __device__ inline Stencil stencil(V3 p) const {
+    Stencil st;
+    const V3 gc = grid_coords(p);
+    st.ok = contains(gc);
+    const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
+    const V3 fl = mk3(floorf(gg.x), floorf(gg.y), floorf(gg.z));
+    st.bx = static_cast<int>(fl.x);
+    st.by = static_cast<int>(fl.y);
+    st.bz = static_cast<int>(fl.z);
+    st.fr = sub3(gg, fl);
+    st.interior = st.bx >= 0 && st.by >= 0 && st.bz >= 0 && st.bx + 1 < g.nx && st.by + 1 < g.ny &&
+                  st.bz + 1 < g.nz;
+    return st;
+  }
+  // Loads of the eight corners (z -> y -> x order); bit c of `inb` marks a
+  // corner inside the grid.
+  __device__ inline void load8(const Stencil& st, typename IO::Raw (&raw)[8], unsigned& inb, SlotCache& cache) const {
+    inb = 0U;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int x = st.bx + (c & 1), y = st.by + ((c >> 1) & 1), z = st.bz + (c >> 2);
+      raw[c] = IO::fresh();
+      if (!st.ok) continue;
+      if (!st.interior && (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz)) continue;
+      raw[c] = s.voxel_cached(g, x, y, z, cache);
+      inb |= 1U << c;
+    }
+  }
+  // blend_tsdf over loaded corners (the same float operations as tsdf()).
+  __device__ inline bool blend(const Stencil& st, const typename IO::Raw (&raw)[8], unsigned inb, bool require_all,
+                               float& out) const {
+    if (!st.ok) return false;
+    float acc = 0.0F, wsum = 0.0F;
+    int valid = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (((inb >> c) & 1U) == 0U) continue;
+      float t, w;
+      bool obs, fin;
+      IO::unpack(raw[c], t, w, obs, fin);
+      if (!obs || !fin) continue;
+      const float cw = axis_w(st.fr.x, c & 1) * axis_w(st.fr.y, (c >> 1) & 1) * axis_w(st.fr.z, c >> 2);
+      acc += cw * t;
+      wsum += cw;
+      ++valid;
+    }
+    if (require_all && valid < 8) return false;
+    if (wsum <= 1.0e-6F) return false;
+    const float d = acc / wsum;
+    if (!isfinite(d)) return false;
+    out = d;
+    return true;
+  }
 
   // TrilinearSampler::tsdf + blend_tsdf.
   __device__ bool tsdf(V3 p, bool require_all, float& out) const {
@@ -156,18 +219,32 @@ struct Sampler {
     return true;
   }
 
-  // central_normal (trilinear.hpp:246-269).
+  // central_normal (trilinear.hpp:246-269).  The six samples' corner loads
+  // are issued together (two at a time for 8-byte voxels, to bound register
+  // use); the blends then run in the reference's order.
   __device__ bool normal(V3 p, bool require_all, V3& out) const {
     float gr[3];
+    constexpr int kBatch = (Store::kDirect && sizeof(typename IO::Raw) == 4) ? 6 : 2;
 #pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-      const V3 off = scale3(mk3(axis == 0 ? 1.0F : 0.0F, axis == 1 ? 1.0F : 0.0F,
-                                axis == 2 ? 1.0F : 0.0F),
-                            g.voxel);
-      float plus, minus;
-      if (!tsdf(add3(p, off), require_all, plus)) return false;
-      if (!tsdf(sub3(p, off), require_all, minus)) return false;
-      gr[axis] = plus - minus;
+    for (int a0 = 0; a0 < 6; a0 += kBatch) {
+      Stencil st[kBatch];
+      typename IO::Raw raw[kBatch][8];
+      unsigned inb[kBatch];
+      SlotCache cache;
+#pragma unroll
+      for (int i = 0; i < kBatch; ++i) {
+        const int axis = (a0 + i) >> 1;
+        const V3 off = scale3(mk3(axis == 0 ? 1.0F : 0.0F, axis == 1 ? 1.0F : 0.0F, axis == 2 ? 1.0F : 0.0F), g.voxel);
+        st[i] = stencil(((a0 + i) & 1) == 0 ? add3(p, off) : sub3(p, off));
+        load8(st[i], raw[i], inb[i], cache);
+      }
+#pragma unroll
+      for (int i = 0; i < kBatch; i += 2) {
+        float plus, minus;
+        if (!blend(st[i], raw[i], inb[i], require_all, plus)) return false;
+        if (!blend(st[i + 1], raw[i + 1], inb[i + 1], require_all, minus)) return false;
+        gr[(a0 + i) >> 1] = plus - minus;
+      }
     }
     const V3 dir = mk3(gr[0], gr[1], gr[2]);
     const float len = norm3(dir);
