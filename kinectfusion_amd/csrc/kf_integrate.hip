@@ -250,8 +250,11 @@ __device__ inline bool same_raw(Raw a, Raw b) {
 // observe + fuse (tsdf_integration.hpp:184-252) of N voxels of one thread,
 // in phases so that the N pixel-record loads, then the N voxel loads, are in
 // flight together: (1) project, (2) read the pixel records, (3) decide,
-// (4) read the voxels, (5) update.  any: some voxel fused; all_ones: every
-// voxel fused and reads tsdf exactly 1 afterwards (free-space part bits).
+// (4) read the voxels, (5) update.  any: some voxel fused; ones_ok: every
+// fused voxel reads tsdf exactly 1 afterwards; complete: every voxel fused.
+// A free-space part (every voxel observed and reading exactly 1) is known
+// when ones_ok holds for all its voxels and either all were fused now or the
+// part was free before (an unfused voxel keeps its value).
 //
 // Most voxels are decided without IEEE divisions: the projection and the
 // signed distance are first computed with the hardware reciprocal and square
@@ -275,7 +278,7 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
                                   typename VoxelIO<Kind>::Raw* __restrict__ vox, float4* __restrict__ colors,
                                   const int (&xs)[N], const int (&ys)[N], const int (&zs)[N],
                                   const size_t (&vis)[N], const bool (&inside)[N], unsigned int& my_fused,
-                                  unsigned int& my_color, bool& any, bool& all_ones) {
+                                  unsigned int& my_color, bool& any, bool& ones_ok, bool& complete) {
   using IO = VoxelIO<Kind>;
   int pi[N];
   V3 cp[N];
@@ -340,11 +343,12 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
   typename IO::Raw raw[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) raw[k] = mode[k] != 0 ? vox[vis[k]] : IO::fresh();
-  all_ones = true;
+  ones_ok = true;
+  complete = true;
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     if (mode[k] == 0) {
-      all_ones = false;
+      complete = false;
       continue;
     }
     const float w = p[k].z;
@@ -362,7 +366,7 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
     const typename IO::Raw nv = IO::pack(avg, tw);
     if (!same_raw(nv, raw[k])) vox[vis[k]] = nv;
     IO::unpack(nv, t0, w0, obs, fin);
-    all_ones = all_ones && t0 == 1.0F;
+    ones_ok = ones_ok && t0 == 1.0F;
     any = true;
     if (Count) ++my_fused;
     if (Color && mode[k] == 2) {
@@ -387,7 +391,8 @@ __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, con
                                         const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
                                         float4* __restrict__ colors, int x, int y, int bz,
-                                        unsigned int& my_fused, unsigned int& my_color, bool& all_ones) {
+                                        unsigned int& my_fused, unsigned int& my_color, bool& ones_ok,
+                                        bool& complete) {
   int xs[kBrickZ], ys[kBrickZ], zs[kBrickZ];
   size_t vis[kBrickZ];
   bool inside[kBrickZ];
@@ -401,7 +406,7 @@ __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, con
   }
   bool any = false;
   fuse_batch<Kind, Color, Count, kBrickZ>(g, f, w2c, pix, color_img, vox, colors, xs, ys, zs, vis, inside, my_fused,
-                                          my_color, any, all_ones);
+                                          my_color, any, ones_ok, complete);
   return any;
 }
 
@@ -423,7 +428,7 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
                                                          unsigned long long* __restrict__ fused,
                                                          uint32_t* __restrict__ obs_bits, BlockMap bm,
                                                          uint32_t* __restrict__ part_bits) {
-  __shared__ uint32_t parts_ok;
+  __shared__ uint32_t parts_ok, parts_complete;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
   const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
@@ -434,18 +439,26 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
     const uint32_t b = list[li];
     const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
     const int x = bx * kBrickX + tx, y = by * kBrickY + ty;
-    bool any = false, ones = false;
-    if (threadIdx.x == 0) parts_ok = 0xFFU;
+    bool any = false, ones_ok = false, complete = false;
+    if (threadIdx.x == 0) {
+      parts_ok = 0xFFU;
+      parts_complete = 0xFFU;
+    }
     __syncthreads();
     // The brick's rows of 64 voxels span 8 occupancy blocks along x and one
     // along y and z: one ballot per wave marks them (lane 0 publishes).
     if (x < g.nx && y < g.ny) any = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors,
-                                                                           x, y, bz, my_fused, my_color, ones);
-    const unsigned long long om = __ballot(ones);
+                                                                           x, y, bz, my_fused, my_color, ones_ok,
+                                                                           complete);
+    const unsigned long long om = __ballot(ones_ok), cm = __ballot(complete);
     if (tx == 0) {
-      uint32_t ok = 0U;
-      for (int j = 0; j < 8; ++j) ok |= ((om >> (8 * j)) & 0xFFULL) == 0xFFULL ? (1U << j) : 0U;
+      uint32_t ok = 0U, cpl = 0U;
+      for (int j = 0; j < 8; ++j) {
+        ok |= ((om >> (8 * j)) & 0xFFULL) == 0xFFULL ? (1U << j) : 0U;
+        cpl |= ((cm >> (8 * j)) & 0xFFULL) == 0xFFULL ? (1U << j) : 0U;
+      }
       atomicAnd(&parts_ok, ok);
+      atomicAnd(&parts_complete, cpl);
     }
     const unsigned long long mask = __ballot(any);
     if (tx == 0 && mask != 0ULL) {
@@ -465,7 +478,8 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
       for (int j = 0; j < 8; ++j) {
         const int blk_x = bx * 8 + j;
         if (blk_x >= bm.bx) break;
-        set_part(part_bits, bm.flat(blk_x, blk_y, blk_z), part, ((parts_ok >> j) & 1U) != 0U);
+        const bool ok = ((parts_ok >> j) & 1U) != 0U, cpl = ((parts_complete >> j) & 1U) != 0U;
+        if (!ok || cpl) set_part(part_bits, bm.flat(blk_x, blk_y, blk_z), part, ok);  // else unchanged
       }
     }
   }
@@ -725,7 +739,7 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
                                                                 uint32_t* __restrict__ obs_bits, BlockMap bm,
                                                                 uint32_t* __restrict__ part_bits) {
   __shared__ int any_fused;
-  __shared__ int all_ones;
+  __shared__ int all_ok, all_complete;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
   const uint32_t n = sp[1];
@@ -738,7 +752,8 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
     const int bz = static_cast<int>(key / static_cast<uint32_t>(bm.bx * bm.by));
     if (threadIdx.x == 0) {
       any_fused = 0;
-      all_ones = 1;
+      all_ok = 1;
+      all_complete = 1;
     }
     __syncthreads();
     int xs[2], ys[2], zs[2];
@@ -753,18 +768,19 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
       inside[pass] = xs[pass] < g.nx && ys[pass] < g.ny && zs[pass] < g.nz;
       vis[pass] = static_cast<size_t>(slot) * kBlockVoxels + i;
     }
-    bool any = false, ones_all = true;
+    bool any = false, ones_ok = true, complete = true;
     fuse_batch<Kind, Color, Count, 2>(g, f, w2c, pix, color_img, pool, cpool, xs, ys, zs, vis, inside, my_fused,
-                                      my_color, any, ones_all);
+                                      my_color, any, ones_ok, complete);
     if (any) any_fused = 1;
-    if (!ones_all) all_ones = 0;
+    if (!ones_ok) all_ok = 0;
+    if (!complete) all_complete = 0;
     __syncthreads();
     if (threadIdx.x == 0) {
       const int fb = bm.flat(bx, by, bz);
       if (any_fused != 0) atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
       const uint32_t nib = 0xFU << ((fb & 7) * 4);
-      if (all_ones != 0) atomicOr(&part_bits[fb >> 3], nib);
-      else atomicAnd(&part_bits[fb >> 3], ~nib);
+      if (all_ok == 0) atomicAnd(&part_bits[fb >> 3], ~nib);
+      else if (all_complete != 0) atomicOr(&part_bits[fb >> 3], nib);  // else unchanged
     }
     __syncthreads();
   }

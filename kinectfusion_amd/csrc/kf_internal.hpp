@@ -334,10 +334,12 @@ struct kf_context {
   // neighbours all unobserved (the raycaster skips those)
   uint32_t* obs_bits = nullptr;
   uint32_t* skip_bits = nullptr;
-  // free space: 4 bits per block (its y-half x z-half parts) set when every
-  // voxel of the part was fused last and reads tsdf exactly 1; a block's free
-  // bit = all 4 parts set for it and its 26 neighbours (the raycaster steps
-  // through those blocks without loads)
+  // free space: 4 bits per block (its y-half x z-half parts), set while every
+  // voxel of the part is observed and reads tsdf exactly 1 (set when a pass
+  // fuses the whole part to 1, kept while unfused voxels keep their values,
+  // cleared when a fused voxel ends elsewhere); a block's free bit = all 4
+  // parts set for it and its 26 neighbours (the raycaster steps through those
+  // blocks without loads)
   uint32_t* part_bits = nullptr;
   uint32_t* free_bits = nullptr;
   size_t part_words = 0;
