@@ -330,6 +330,14 @@ kf_status kf_hashed_stats(kf_context* ctx, int64_t* stats);
  * in increasing order; *count receives their number. */
 kf_status kf_hashed_blocks(kf_context* ctx, uint32_t* keys, int64_t capacity, int64_t* count);
 
+/* ---- block maps used to skip samples (no counterpart in the reference; a
+ * diagnostic): one bit per 8^3 voxel block, flat x-major block order.
+ * observed: some voxel of the block is observed; skip: the block and its 26
+ * neighbours are unobserved; free_space: the block and its 26 neighbours
+ * hold only observed voxels reading tsdf exactly 1.  *words receives the
+ * number of 32-bit words of each map; NULL maps are not copied. */
+kf_status kf_block_bitmaps(kf_context* ctx, uint32_t* observed, uint32_t* skip, uint32_t* free_space, int64_t* words);
+
 /* ---- MarchingCubes::extract (marching_cubes.cpp:478-507) on the device:
  * cell classification, stream-compacted triangles and the reference's
  * first-encounter vertex welding order. */

@@ -134,6 +134,7 @@ HIP_SYMBOLS = [
     ("kf_mesh_size", _i32, [_vp, C.POINTER(_i64), C.POINTER(_i64)]),
     ("kf_mesh_copy", _i32, [_vp, _vp, _vp, _vp, _vp]),
     ("kf_mesh_free", None, [_vp]),
+    ("kf_block_bitmaps", _i32, [_vp, _vp, _vp, _vp, C.POINTER(_i64)]),
     ("kf_hashed_stats", _i32, [_vp, C.POINTER(_i64)]),
     ("kf_hashed_blocks", _i32, [_vp, _vp, _i64, C.POINTER(_i64)]),
     ("kf_posenet_config_defaults", None, [C.POINTER(KfPosenetConfig)]),

@@ -1047,6 +1047,20 @@ extern "C" kf_status kf_hashed_blocks(kf_context* ctx, uint32_t* keys, int64_t c
   });
 }
 
+extern "C" kf_status kf_block_bitmaps(kf_context* ctx, uint32_t* observed, uint32_t* skip, uint32_t* free_space,
+                                      int64_t* words) {
+  return guard(ctx, [&] {
+    if (words != nullptr) *words = static_cast<int64_t>(ctx->block_words);
+    if (observed != nullptr)
+      KF_HIP(hipMemcpyAsync(observed, ctx->obs_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (skip != nullptr)
+      KF_HIP(hipMemcpyAsync(skip, ctx->skip_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (free_space != nullptr)
+      KF_HIP(hipMemcpyAsync(free_space, ctx->free_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
 extern "C" kf_status kf_attach_posenet(kf_context* ctx, kf_posenet* net) {
   return guard(ctx, [&] {
     if (net != nullptr) {

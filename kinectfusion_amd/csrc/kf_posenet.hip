@@ -93,8 +93,23 @@ std::vector<Shape> network_shapes(size_t* total) {
 // the four waves a 32x32 quadrant of 2x2 16x16x16 MFMA blocks, K staged 32
 // at a time through LDS.  gridDim.z splits K; split partials go to
 // `partial` and a second kernel sums them in split order.
+// A 3x3 stride-2 pad-1 convolution read as its im2col matrix, element
+// (m, k) = x[b][2 yo - 1 + ky][2 xo - 1 + kx][c] with m = (b Ho + yo) Wo + xo,
+// k = (ky 3 + kx) C + c (0 outside the image).  With `raw_prev` set the input
+// is the depth pair itself (u16, channel 0 previous, 1 current), converted as
+// the input layer does.
+struct ConvView {
+  const uint16_t* x;
+  const uint16_t* raw_prev;  // pair b at raw_prev + b * raw_stride
+  const uint16_t* raw_cur;
+  long long raw_stride;
+  float inv_scale;
+  int H, W, C, Ho, Wo;
+};
+
 struct Gemm {
-  const uint16_t* a;
+  const uint16_t* a;  // null: A is the im2col matrix of `conv`
+  ConvView conv;
   long long sam, sak;
   const uint16_t* b;
   long long sbk, sbn;
@@ -107,14 +122,29 @@ struct Gemm {
   long long ofm, ofn;
   uint16_t* outh;
   long long ohm, ohn;
+  int scale_out;  // outf receives v * out_scale(n) (the twist)
 };
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 4;
 
+__device__ inline uint16_t conv_element(const ConvView& v, int m, int k) {
+  const int tap = k / v.C, c = k - tap * v.C;
+  const int ky = tap / 3, kx = tap - ky * 3;
+  const int xo = m % v.Wo, t = m / v.Wo;
+  const int yo = t % v.Ho, b = t / v.Ho;
+  const int yi = 2 * yo - 1 + ky, xi = 2 * xo - 1 + kx;
+  if (yi < 0 || yi >= v.H || xi < 0 || xi >= v.W) return 0;
+  if (v.raw_prev != nullptr) {
+    const uint16_t r = (c == 0 ? v.raw_prev : v.raw_cur)[b * v.raw_stride + yi * v.W + xi];
+    return to_bf16(r == 0 ? 0.0F : static_cast<float>(r) * v.inv_scale - kDepthCentre);
+  }
+  return v.x[((static_cast<long long>(b) * v.H + yi) * v.W + xi) * v.C + c];
+}
+
 __device__ inline void gemm_store(const Gemm& g, int m, int n, float v) {
   if (g.bias != nullptr) v += g.bias[n];
   if (g.relu != 0) v = fmaxf(v, 0.0F);
-  if (g.outf != nullptr) g.outf[m * g.ofm + n * g.ofn] = v;
+  if (g.outf != nullptr) g.outf[m * g.ofm + n * g.ofn] = g.scale_out != 0 ? v * out_scale(n) : v;
   if (g.outh != nullptr) g.outh[m * g.ohm + n * g.ohn] = to_bf16(v);
 }
 
@@ -135,7 +165,8 @@ __global__ void __launch_bounds__(256) gemm_kernel(Gemm g) {
       const int r = e / BK, c = e % BK;
       const int k = k0 + c;
       const int m = m0 + r, n = n0 + r;
-      As[r][c] = (m < g.M && k < ke) ? g.a[m * g.sam + k * g.sak] : static_cast<uint16_t>(0);
+      As[r][c] = (m < g.M && k < ke) ? (g.a != nullptr ? g.a[m * g.sam + k * g.sak] : conv_element(g.conv, m, k))
+                                     : static_cast<uint16_t>(0);
       Bs[r][c] = (n < g.N && k < ke) ? g.b[k * g.sbk + n * g.sbn] : static_cast<uint16_t>(0);
     }
     __syncthreads();
@@ -330,10 +361,6 @@ __global__ void to_bf16_kernel(const float* w, uint16_t* wh, long long n) {
   if (i < n) wh[i] = to_bf16(w[i]);
 }
 
-__global__ void scale_out_kernel(const float* out, float* twist, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) twist[i] = out[i] * out_scale(i % kOut);
-}
 
 int blocks_for(long long n, int per = 256) { return static_cast<int>((n + per - 1) / per); }
 
@@ -377,7 +404,7 @@ void gemm(kf_posenet* net, hipStream_t s, Gemm g) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = tm * tn;
   int splits = 1;
-  if (tiles < 256 && g.K > 4 * BK) {
+  if (tiles < 256 && g.K > 1024) {  // only the long reductions (weight gradients, fc1) split
     splits = std::min((g.K + BK - 1) / BK, (512 + tiles - 1) / tiles);
   }
   int kchunk = (g.K + splits - 1) / splits;
@@ -573,20 +600,52 @@ void init_weights(const std::vector<Shape>& shapes, uint64_t seed, std::vector<f
   }
 }
 
+// Inference forward pass for `batch` depth pairs (pair b: previous at
+// prev + b * stride, current at cur + b * stride, 160 x 120 u16 each) ->
+// twist[batch][6] (omega rad, t m).  Eight launches: the convolutions read
+// their input as an implicit im2col matrix (the first one straight from the
+// depth pairs) and the last layer writes the scaled twist.
+void infer(kf_posenet* net, hipStream_t s, const uint16_t* prev, const uint16_t* cur, long long stride, int batch,
+           float* twist) {
+  for (int l = 0; l < kConv; ++l) {
+    const Shape& sh = net->shapes[l];
+    const int M = batch * pixels(sh), K = sh.fan_in;
+    Gemm g = make_gemm(nullptr, 0, 0, net->wh + sh.w_off, 1, K, M, sh.cout, K);
+    g.conv = ConvView{net->act[l],   l == 0 ? prev : nullptr, l == 0 ? cur : nullptr, stride,
+                      1.0F / net->cfg.depth_scale, sh.hin, sh.win, sh.cin, sh.hout, sh.wout};
+    g.bias = net->w + sh.b_off;
+    g.relu = 1;
+    g.outh = net->act[l + 1];
+    g.ohm = sh.cout;
+    g.ohn = 1;
+    gemm(net, s, g);
+  }
+  const Shape& f1 = net->shapes[kConv];
+  Gemm g1 = make_gemm(net->act[kConv], f1.fan_in, 1, net->wh + f1.w_off, 1, f1.fan_in, batch, f1.cout, f1.fan_in);
+  g1.bias = net->w + f1.b_off;
+  g1.relu = 1;
+  g1.outh = net->act[kConv + 1];
+  g1.ohm = f1.cout;
+  g1.ohn = 1;
+  gemm(net, s, g1);
+  const Shape& f2 = net->shapes[kConv + 1];
+  Gemm g2 = make_gemm(net->act[kConv + 1], f2.fan_in, 1, net->wh + f2.w_off, 1, f2.fan_in, batch, f2.cout,
+                      f2.fan_in);
+  g2.bias = net->w + f2.b_off;
+  g2.outf = twist;
+  g2.ofm = kOut;
+  g2.ofn = 1;
+  g2.scale_out = 1;
+  gemm(net, s, g2);
+}
+
 }  // namespace
 
 namespace kf {
 
-// The network's forward pass on the pipeline's stream for one pair: the
-// previous and current coarsest-level depth (160 x 120 u16 each) -> six
-// floats of twist (omega rad, t m) in `twist`.
+// The pose proposal on the pipeline's stream for one pair.
 void posenet_propose(kf_posenet* net, hipStream_t s, const uint16_t* prev, const uint16_t* cur, float* twist) {
-  const long long hw = static_cast<long long>(kInH) * kInW;
-  hipLaunchKernelGGL(input_kernel, dim3(blocks_for(hw), 1), dim3(256), 0, s, prev, cur, 0LL, 1,
-                     1.0F / net->cfg.depth_scale, net->act[0]);
-  forward(net, s, 1);
-  hipLaunchKernelGGL(scale_out_kernel, dim3(1), dim3(64), 0, s, net->out, twist, kOut);
-  KF_HIP(hipGetLastError());
+  infer(net, s, prev, cur, 0LL, 1, twist);
 }
 
 bool posenet_input_matches(int width, int height) { return width == kInW && height == kInH; }
@@ -721,10 +780,8 @@ kf_status kf_posenet_forward(kf_posenet* net, const uint16_t* pairs, int32_t bat
   return net_guard(net, [&] {
     if (net == nullptr || pairs == nullptr || twists == nullptr) fail(KF_ERR_INVALID_ARGUMENT, "null argument");
     check_batch(net, batch);
-    prepare_pairs(net, net->stream, pairs, batch);
-    forward(net, net->stream, batch);
-    hipLaunchKernelGGL(scale_out_kernel, dim3(blocks_for(batch * kOut)), dim3(256), 0, net->stream, net->out,
-                       twists, batch * kOut);
+    const long long hw = static_cast<long long>(kInH) * kInW;
+    infer(net, net->stream, pairs, pairs + hw, 2 * hw, batch, twists);
     KF_HIP(hipGetLastError());
     KF_HIP(hipStreamSynchronize(net->stream));
   });

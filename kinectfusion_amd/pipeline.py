@@ -368,6 +368,19 @@ class Pipeline:
         self._check(N.hip().kf_attach_posenet(self._ctx, net.handle if net is not None else None))
         self._posenet = net
 
+    def block_bitmaps(self) -> dict:
+        """The per-block observed / skip / free-space bits as bool arrays in
+        flat block order."""
+        words = C.c_int64()
+        self._check(N.hip().kf_block_bitmaps(self._ctx, None, None, None, C.byref(words)))
+        maps = [np.zeros(words.value, np.uint32) for _ in range(3)]
+        self._check(N.hip().kf_block_bitmaps(self._ctx, *[m.ctypes.data_as(C.c_void_p) for m in maps],
+                                             C.byref(words)))
+        r = self.config.volume.resolution
+        nblocks = ((r[0] + 7) // 8) * ((r[1] + 7) // 8) * ((r[2] + 7) // 8)
+        bits = [np.unpackbits(m.view(np.uint8), bitorder="little")[:nblocks].astype(bool) for m in maps]
+        return {"observed": bits[0], "skip": bits[1], "free": bits[2]}
+
     def hashed_stats(self) -> dict:
         """Hashed volume: allocated blocks, dropped requests (overflow), hash
         entries, blocks integrated by the last frame."""

@@ -440,3 +440,34 @@ def test_captured_frames_match_eager_frames(K, scene, torch_gpu):
     eager, graph = run(True), run(False)
     assert np.array_equal(eager[0], graph[0])
     assert np.array_equal(eager[1], graph[1]) and np.array_equal(eager[2], graph[2])
+
+
+@pytest.mark.parametrize("store", ["float", "bf16"])
+def test_free_space_blocks_hold_only_ones(K, scene, torch_gpu, store):
+    """A block the raycaster steps through as free space must hold, with its
+    26 neighbours, only observed voxels reading exactly 1; and the marking
+    finds most such blocks (256^3 of 1 cm, after 8 tracked frames)."""
+    from scipy.ndimage import binary_erosion
+
+    res = 256
+    p = K.Pipeline(pipeline_config(K, res=res, store=store, voxel=0.01))
+    for i in range(8):
+        depth, rgba, _ = scene.frame(i)
+        p.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba))
+    maps = p.block_bitmaps()
+    vox, _ = p.host_view()
+    nb = res // 8
+    if store == "float":
+        t, w = vox[:, 0], vox[:, 1]
+    else:
+        t = (vox[:, 0].astype(np.uint32) << 16).view(np.float32)
+        w = vox[:, 1].astype(np.float32)
+    ones = ((t == 1.0) & (w > 0)).reshape(res, res, res)  # z, y, x
+    blocks_ones = ones.reshape(nb, 8, nb, 8, nb, 8).all(axis=(1, 3, 5))  # bz, by, bx
+    possible = binary_erosion(blocks_ones, structure=np.ones((3, 3, 3), bool), border_value=0)
+    free = maps["free"].reshape(nb, nb, nb)
+    print(f"{store}: observed blocks {maps['observed'].sum()}, all-ones blocks {blocks_ones.sum()}, "
+          f"free-able {possible.sum()}, free {free.sum()}")
+    assert not np.any(free & ~possible)  # every free block really is free space
+    assert possible.sum() > 0 and free.sum() >= 0.5 * possible.sum()
+    p.close()
