@@ -584,8 +584,6 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->fused = dalloc<unsigned long long>(4);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     ctx->partials = dalloc<float>(512 * 32);
-    ctx->ticket = dalloc<unsigned int>(1);
-    KF_HIP(hipMemsetAsync(ctx->ticket, 0, sizeof(unsigned int), ctx->stream));
     ctx->icp = dalloc<IcpState>(1);
     KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
     KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));
@@ -641,7 +639,6 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->counters);
   dfree(ctx->fused);
   dfree(ctx->partials);
-  dfree(ctx->ticket);
   dfree(ctx->icp);
   dfree(ctx->trajectory);
   dfree(ctx->prev_coarse);

@@ -7,15 +7,18 @@
 // (src/kinectfusion/icp_optimizer.cpp:82-169).
 //
 // Two kernels per Gauss-Newton / LM trial:
-//  * every thread matches its pixels and keeps the 29 per-correspondence
-//    terms (21 J^T J, 6 J^T r, distance, r^2) in FP32 registers;
+//  * every thread matches its pixels (five at a time, their memory reads in
+//    flight together) and keeps the 29 per-correspondence terms (21 J^T J,
+//    6 J^T r, distance, r^2) in FP32 registers;
 //  * a wave reduces them with an xor butterfly, the four waves of a
 //    workgroup combine in a fixed order through LDS, and each workgroup
 //    writes one partial row -- no float atomics, so the sums do not depend
 //    on scheduling;
-//  * the last workgroup to finish (ticket counter) adds the partial rows in
-//    a fixed order with Kahan-compensated FP32;
-//  * a one-wave kernel converts the sums to FP64 and runs the trial: FP64
+//  * a one-workgroup kernel adds the partial rows in a fixed order with
+//    Kahan-compensated FP32 (a kernel boundary, not a device-scope fence in
+//    every sweep workgroup, makes the rows visible: such a fence writes back
+//    the L2 and costs more than the sweep itself);
+//  * one thread of it converts the sums to FP64 and runs the trial: FP64
 //    pivoted LDL^T solve, caps, Rodrigues increment and pose composition in
 //    FP64, convergence and failure latches.
 // Once a level's status latches (converged or failed) the remaining trial
@@ -40,23 +43,28 @@ struct Maps {
   Intr k;
 };
 
-// match() for one live pixel; returns false when rejected.
-__device__ inline bool match_pixel(const Maps& m, const Rigid& cam, const Rigid& mod, float max_dist,
-                                   float min_dot, int x, int y, float* J, float& res, float& dist) {
-  const size_t li = static_cast<size_t>(y) * m.lw + x;
-  const V3 lv = ld3(m.lv[li]);
-  const V3 ln = ld3(m.ln[li]);
+// match() (icp_correspondence.hpp:44-110) in two halves so that a thread
+// can keep the memory reads of several pixels in flight together.
+// Front: read the live vertex and normal, transform, project into the model
+// maps; returns false when rejected before the model read.
+__device__ inline bool match_front(const Maps& m, const Rigid& cam, const Rigid& mod, V3 lv, V3 ln, V3& src,
+                                   size_t& mi) {
   if (!finite3(lv) || !finite3(ln)) return false;
-  const V3 src = xform(cam, lv);
+  src = xform(cam, lv);
   const V3 sim = xform(mod, src);
   if (!finite3(sim) || sim.z <= 0.0F) return false;
   float u, v;
   project(m.k, sim, u, v);
   int col, row;
   if (!round_index(u, m.mw, col) || !round_index(v, m.mh, row)) return false;
-  const size_t mi = static_cast<size_t>(row) * m.mw + col;
-  const V3 mv = ld3(m.mv[mi]);
-  const V3 mn = ld3(m.mn[mi]);
+  mi = static_cast<size_t>(row) * m.mw + col;
+  return true;
+}
+
+// Back: with the model vertex and normal read, the distance and normal
+// tests and the point-to-plane row (J, residual).
+__device__ inline bool match_back(const Rigid& cam, float max_dist, float min_dot, V3 ln, V3 src, V3 mv, V3 mn,
+                                  float* J, float& res, float& dist) {
   if (!finite3(mv) || !finite3(mn)) return false;
   const V3 sn = normalize3(rot(cam, ln));
   dist = norm3(sub3(src, mv));
@@ -73,6 +81,11 @@ __device__ inline bool match_pixel(const Maps& m, const Rigid& cam, const Rigid&
   return true;
 }
 
+// Pixels a thread matches per pass (grid-stride order): their live reads,
+// then their model reads, are issued together; the terms are added in pixel
+// order.
+constexpr int kPixBatch = 5;
+
 // Per-workgroup sweep: sums of this block's pixels into partial[blockIdx.x].
 __device__ inline void sweep_block(const Maps& m, const Rigid& cam, const Rigid& mod, float max_dist,
                                    float min_dot, float* __restrict__ partial, float (*lds)[kSums + 1]) {
@@ -81,19 +94,53 @@ __device__ inline void sweep_block(const Maps& m, const Rigid& cam, const Rigid&
   for (int i = 0; i < kSums; ++i) s[i] = 0.0F;
   unsigned int cnt = 0;
   const int n = m.lw * m.lh;
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    float J[6], res, dist;
-    if (!match_pixel(m, cam, mod, max_dist, min_dot, p % m.lw, p / m.lw, J, res, dist)) continue;
-    int e = 0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int base = blockIdx.x * blockDim.x + threadIdx.x; base < n; base += kPixBatch * stride) {
+    V3 lv[kPixBatch], ln[kPixBatch];
+    bool ok[kPixBatch];
 #pragma unroll
-    for (int r = 0; r < 6; ++r)
+    for (int j = 0; j < kPixBatch; ++j) {
+      const int p = base + j * stride;
+      ok[j] = p < n;
+      lv[j] = mk3(0.0F, 0.0F, 0.0F);
+      ln[j] = lv[j];
+      if (ok[j]) {
+        lv[j] = ld3(m.lv[p]);
+        ln[j] = ld3(m.ln[p]);
+      }
+    }
+    V3 src[kPixBatch], mv[kPixBatch], mn[kPixBatch];
+    size_t mi[kPixBatch];
 #pragma unroll
-      for (int c = r; c < 6; ++c, ++e) s[e] += J[r] * J[c];
+    for (int j = 0; j < kPixBatch; ++j) {
+      src[j] = mk3(0.0F, 0.0F, 0.0F);
+      mi[j] = 0;
+      ok[j] = ok[j] && match_front(m, cam, mod, lv[j], ln[j], src[j], mi[j]);
+    }
 #pragma unroll
-    for (int r = 0; r < 6; ++r) s[21 + r] += J[r] * res;
-    s[27] += dist;
-    s[28] += res * res;
-    ++cnt;
+    for (int j = 0; j < kPixBatch; ++j) {
+      mv[j] = mk3(0.0F, 0.0F, 0.0F);
+      mn[j] = mv[j];
+      if (ok[j]) {
+        mv[j] = ld3(m.mv[mi[j]]);
+        mn[j] = ld3(m.mn[mi[j]]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kPixBatch; ++j) {
+      float J[6], res, dist;
+      if (!ok[j] || !match_back(cam, max_dist, min_dot, ln[j], src[j], mv[j], mn[j], J, res, dist)) continue;
+      int e = 0;
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = r; c < 6; ++c, ++e) s[e] += J[r] * J[c];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) s[21 + r] += J[r] * res;
+      s[27] += dist;
+      s[28] += res * res;
+      ++cnt;
+    }
   }
   // wave butterfly
 #pragma unroll
@@ -119,32 +166,34 @@ __device__ inline void sweep_block(const Maps& m, const Rigid& cam, const Rigid&
   }
 }
 
-// Ticket protocol: returns true in the workgroup that finished last, after
-// which the partial rows of every workgroup are visible.
-__device__ inline bool last_block(unsigned int* ticket) {
-  __shared__ bool is_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned int t = atomicAdd(ticket, 1U);
-    is_last = (t == gridDim.x - 1U);
-  }
-  __syncthreads();
-  if (is_last) __threadfence();
-  return is_last;
-}
-
 // Kahan-compensated FP32 sum of the partial rows.  The rows are first
 // staged in LDS with coalesced loads.  Column c (29 sums and the count) is
 // split over 8 threads: part q adds rows q, q + 8, ... with Kahan, then
 // thread c adds the 8 part sums (net of their compensations) in order
 // q = 0..7.  The order is fixed, so the result does not depend on scheduling.
 __device__ inline void combine_partials(const float* __restrict__ partial, int nblocks, float* out) {
-  __shared__ float rows[kMaxBlocks * kPartialStride];
+  __shared__ __align__(16) float rows[kMaxBlocks * kPartialStride];
   __shared__ float part_sum[8][kPartialStride];
   __shared__ float part_comp[8][kPartialStride];
   __shared__ unsigned part_cnt[8];
-  for (int i = threadIdx.x; i < nblocks * kPartialStride; i += blockDim.x) rows[i] = partial[i];
+  {
+    // every row in flight at once (8 float4 per thread), not one round trip
+    // per loop iteration
+    constexpr int kQuads = kMaxBlocks * kPartialStride / 4 / 256;
+    const int quads = nblocks * kPartialStride / 4;
+    const float4* src = reinterpret_cast<const float4*>(partial);
+    float4 v[kQuads];
+#pragma unroll
+    for (int j = 0; j < kQuads; ++j) {
+      const int i = static_cast<int>(threadIdx.x) + j * 256;
+      v[j] = i < quads ? src[i] : make_float4(0.0F, 0.0F, 0.0F, 0.0F);
+    }
+#pragma unroll
+    for (int j = 0; j < kQuads; ++j) {
+      const int i = static_cast<int>(threadIdx.x) + j * 256;
+      if (i < quads) reinterpret_cast<float4*>(rows)[i] = v[j];
+    }
+  }
   __syncthreads();
   const int col = threadIdx.x & 31, q = threadIdx.x >> 5;  // 256 threads: 32 columns x 8 parts
   float sum = 0.0F, comp = 0.0F;
@@ -396,13 +445,10 @@ __device__ void run_trial(IcpState* st, const DeviceIcpParams& p, const float* s
   if (accepted && ut < p.min_t && ang < p.min_r) st->status = 1;
 }
 
-// Correspondence sweep of one trial: sums land in st->sums32.  The solve
-// runs in icp_solve_kernel so that this kernel stays small (high occupancy).
+// One trial's correspondence sweep: one partial row per workgroup.
 __global__ void __launch_bounds__(256) icp_trial_sweep_kernel(Maps m, DeviceIcpParams p, IcpState* st,
-                                                               float* __restrict__ partial,
-                                                               unsigned int* __restrict__ ticket) {
+                                                               float* __restrict__ partial) {
   __shared__ float lds[4][kSums + 1];
-  __shared__ float combined[kSums + 1];
   __shared__ float xf[24];
   if (st->status != 0 || st->frame_failed) return;  // latched: nothing to do
   if (threadIdx.x == 0) {
@@ -416,35 +462,36 @@ __global__ void __launch_bounds__(256) icp_trial_sweep_kernel(Maps m, DeviceIcpP
   __syncthreads();
   const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
   sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
-  if (!last_block(ticket)) return;
-  combine_partials(partial, gridDim.x, combined);
-  __syncthreads();
-  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
-  if (threadIdx.x == 0) *ticket = 0U;
 }
 
-// The trial's FP64 solve and pose update (one thread).
-__global__ void __launch_bounds__(64) icp_solve_kernel(DeviceIcpParams p, IcpState* st) {
-  if (threadIdx.x != 0) return;
-  if (st->status != 0 || st->frame_failed) return;
-  float sums[kSums + 1];
-  for (int i = 0; i <= kSums; ++i) sums[i] = st->sums32[i];
-  run_trial(st, p, sums);
-}
-
-// Single sweep with explicit float transforms (kf_icp_reduce).
-__global__ void __launch_bounds__(256) icp_sweep_kernel(Maps m, Rigid cam, Rigid mod, float max_dist,
-                                                         float min_dot, IcpState* st,
-                                                         float* __restrict__ partial,
-                                                         unsigned int* __restrict__ ticket) {
-  __shared__ float lds[4][kSums + 1];
+// The trial's reduction (one workgroup: the partial rows into st->sums32)
+// and its FP64 solve and pose update (one thread).  A kernel of its own, so
+// the partial rows of every sweep workgroup are visible without a
+// device-scope fence in each of them.
+__global__ void __launch_bounds__(256) icp_trial_solve_kernel(DeviceIcpParams p, IcpState* st,
+                                                               const float* __restrict__ partial, int nblocks) {
   __shared__ float combined[kSums + 1];
-  sweep_block(m, cam, mod, max_dist, min_dot, partial, lds);
-  if (!last_block(ticket)) return;
-  combine_partials(partial, gridDim.x, combined);
+  if (st->status != 0 || st->frame_failed) return;
+  combine_partials(partial, nblocks, combined);
   __syncthreads();
   if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
-  if (threadIdx.x == 0) *ticket = 0U;
+  if (threadIdx.x == 0) run_trial(st, p, combined);
+}
+
+// Single sweep with explicit float transforms (kf_icp_reduce), then its
+// reduction into st->sums32.
+__global__ void __launch_bounds__(256) icp_sweep_kernel(Maps m, Rigid cam, Rigid mod, float max_dist,
+                                                         float min_dot, float* __restrict__ partial) {
+  __shared__ float lds[4][kSums + 1];
+  sweep_block(m, cam, mod, max_dist, min_dot, partial, lds);
+}
+
+__global__ void __launch_bounds__(256) icp_combine_kernel(IcpState* st, const float* __restrict__ partial,
+                                                           int nblocks) {
+  __shared__ float combined[kSums + 1];
+  combine_partials(partial, nblocks, combined);
+  __syncthreads();
+  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
 }
 
 // FP64 cyclic Jacobi eigenvalues of the symmetric 6x6 system.
@@ -590,8 +637,8 @@ void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, i
   const int nb = blocks_for(lw * lh);
   for (int it = 0; it < iterations; ++it) {
     hipLaunchKernelGGL(icp_trial_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
-                       ctx->partials, ctx->ticket);
-    hipLaunchKernelGGL(icp_solve_kernel, dim3(1), dim3(64), 0, ctx->stream, p, ctx->icp);
+                       ctx->partials);
+    hipLaunchKernelGGL(icp_trial_solve_kernel, dim3(1), dim3(256), 0, ctx->stream, p, ctx->icp, ctx->partials, nb);
   }
   hipLaunchKernelGGL(icp_finish_level_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, p, level);
   KF_HIP(hipGetLastError());
@@ -604,8 +651,8 @@ void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* l
   const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
   const int nb = blocks_for(lw * lh);
   hipLaunchKernelGGL(icp_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m,
-                     rigid_from12(camera12), rigid_from12(model12), max_dist, min_dot, ctx->icp,
-                     ctx->partials, ctx->ticket);
+                     rigid_from12(camera12), rigid_from12(model12), max_dist, min_dot, ctx->partials);
+  hipLaunchKernelGGL(icp_combine_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->icp, ctx->partials, nb);
   KF_HIP(hipGetLastError());
 }
 

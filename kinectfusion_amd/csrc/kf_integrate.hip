@@ -251,10 +251,12 @@ __device__ inline bool same_raw(Raw a, Raw b) {
 // in phases so that the N pixel-record loads, then the N voxel loads, are in
 // flight together: (1) project, (2) read the pixel records, (3) decide,
 // (4) read the voxels, (5) update.  any: some voxel fused; ones_ok: every
-// fused voxel reads tsdf exactly 1 afterwards; complete: every voxel fused.
-// A free-space part (every voxel observed and reading exactly 1) is known
-// when ones_ok holds for all its voxels and either all were fused now or the
-// part was free before (an unfused voxel keeps its value).
+// fused voxel reads tsdf exactly 1 afterwards; complete: every voxel was
+// fused, or sits at an in-image pixel without depth and already holds an
+// observed 1.  A free-space part (every voxel observed and reading exactly
+// 1) is known when ones_ok holds for all its voxels and either all are
+// complete now or the part was free before (an unfused voxel keeps its
+// value).
 //
 // Most voxels are decided without IEEE divisions: the projection and the
 // signed distance are first computed with the hardware reciprocal and square
@@ -312,14 +314,20 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
   float4 p[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) p[k] = pi[k] >= 0 ? pix[pi[k]] : make_float4(-1.0F, 1.0F, 0.0F, 1.0F);
-  // mode: 0 no fuse, 1 free space (tsdf 1, no colour), 2 in the band (exact sd)
+  // mode: 0 no fuse, 1 free space (tsdf 1, no colour), 2 in the band (exact
+  // sd), 3 no fuse at an in-image pixel without depth: the voxel is read
+  // only to learn whether it already holds an observed 1 (a depth dropout
+  // inside free space then does not break the part's completeness).
   int mode[N];
   float sd[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     mode[k] = 0;
     sd[k] = 0.0F;
-    if (p[k].x < 0.0F || p[k].z <= 0.0F) continue;  // no depth, or w <= 0 drops the fuse
+    if (p[k].x < 0.0F || p[k].z <= 0.0F) {  // no depth, or w <= 0 drops the fuse
+      if (pi[k] >= 0) mode[k] = 3;
+      continue;
+    }
     if (f.projective) {
       const float sa = __builtin_amdgcn_sqrtf(dot3(cp[k], cp[k])) * __builtin_amdgcn_rcpf(p[k].y);
       const float sda = p[k].x - sa;
@@ -355,6 +363,10 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
     float t0, w0;
     bool obs, fin;
     IO::unpack(raw[k], t0, w0, obs, fin);
+    if (mode[k] == 3) {
+      complete = complete && obs && t0 == 1.0F;
+      continue;
+    }
     float avg;
     if (mode[k] == 1) {
       avg = t0 == 1.0F ? 1.0F : ((t0 * w0) + (1.0F * w)) / (w0 + w);

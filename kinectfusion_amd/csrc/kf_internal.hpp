@@ -348,7 +348,6 @@ struct kf_context {
   unsigned long long* fused = nullptr;
   // icp scratch
   float* partials = nullptr;      // per-block partial sums (32 floats each)
-  unsigned int* ticket = nullptr;
   kf::IcpState* icp = nullptr;    // device
   kf::IcpState* icp_host = nullptr;  // pinned mirror for results
   double* trajectory = nullptr;   // device, 16 doubles per frame

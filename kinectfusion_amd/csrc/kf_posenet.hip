@@ -91,8 +91,9 @@ std::vector<Shape> network_shapes(size_t* total) {
 // GEMM: C(m, n) = sum_k A(m, k) B(k, n), bf16 operands with arbitrary
 // strides, FP32 accumulation.  64x64 tile per 256-thread workgroup, each of
 // the four waves a 32x32 quadrant of 2x2 16x16x16 MFMA blocks, K staged 32
-// at a time through LDS.  gridDim.z splits K; split partials go to
-// `partial` and a second kernel sums them in split order.
+// at a time through LDS.  gridDim.z splits K: every split writes its FP32
+// partial tile and a second kernel adds the partials in split order 0, 1,
+// ... and applies the epilogue, so the sum does not depend on timing.
 // A 3x3 stride-2 pad-1 convolution read as its im2col matrix, element
 // (m, k) = x[b][2 yo - 1 + ky][2 xo - 1 + kx][c] with m = (b Ho + yo) Wo + xo,
 // k = (ky 3 + kx) C + c (0 outside the image).  With `raw_prev` set the input
@@ -114,7 +115,7 @@ struct Gemm {
   const uint16_t* b;
   long long sbk, sbn;
   int M, N, K, kchunk;
-  float* partial;
+  float* partial;  // [splits][M][N] when gridDim.z > 1
   // epilogue: v (+ bias[n]) (ReLU) -> outf and/or outh
   const float* bias;
   int relu;
@@ -126,19 +127,12 @@ struct Gemm {
 };
 
 constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 4;
+constexpr int kStageRows = 256 / BK;           // rows one pass of the workgroup stages
+constexpr int kRowsPerThread = BM / kStageRows;  // 8
+constexpr int kMaxTiles = 256;                   // at most this many tiles split K
 
-__device__ inline uint16_t conv_element(const ConvView& v, int m, int k) {
-  const int tap = k / v.C, c = k - tap * v.C;
-  const int ky = tap / 3, kx = tap - ky * 3;
-  const int xo = m % v.Wo, t = m / v.Wo;
-  const int yo = t % v.Ho, b = t / v.Ho;
-  const int yi = 2 * yo - 1 + ky, xi = 2 * xo - 1 + kx;
-  if (yi < 0 || yi >= v.H || xi < 0 || xi >= v.W) return 0;
-  if (v.raw_prev != nullptr) {
-    const uint16_t r = (c == 0 ? v.raw_prev : v.raw_cur)[b * v.raw_stride + yi * v.W + xi];
-    return to_bf16(r == 0 ? 0.0F : static_cast<float>(r) * v.inv_scale - kDepthCentre);
-  }
-  return v.x[((static_cast<long long>(b) * v.H + yi) * v.W + xi) * v.C + c];
+__device__ inline uint16_t raw_to_input(uint16_t r, float inv_scale) {
+  return to_bf16(r == 0 ? 0.0F : static_cast<float>(r) * inv_scale - kDepthCentre);
 }
 
 __device__ inline void gemm_store(const Gemm& g, int m, int n, float v) {
@@ -157,17 +151,60 @@ __global__ void __launch_bounds__(256) gemm_kernel(Gemm g) {
   const int ke = min(g.K, kb + g.kchunk);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
   const int li = lane & 15, lq = lane >> 4;
+  // This thread stages column `sc` of rows sr + kStageRows * i of both tiles.
+  const int sc = tid & (BK - 1), sr = tid >> 5;
+  const bool implicit = g.a == nullptr;
+  // Implicit im2col: the receptive-field origin of each staged row is fixed
+  // over the K loop (rows past M get an origin no tap reaches).
+  int oy[kRowsPerThread], ox[kRowsPerThread], ob[kRowsPerThread];
+#pragma unroll
+  for (int i = 0; i < kRowsPerThread; ++i) {
+    oy[i] = -4;
+    ox[i] = -4;
+    ob[i] = 0;
+    const int m = m0 + sr + kStageRows * i;
+    if (implicit && m < g.M) {
+      const int xo = m % g.conv.Wo, t = m / g.conv.Wo;
+      oy[i] = 2 * (t % g.conv.Ho) - 1;
+      ox[i] = 2 * xo - 1;
+      ob[i] = t / g.conv.Ho;
+    }
+  }
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
   for (int k0 = kb; k0 < ke; k0 += BK) {
-    for (int e = tid; e < BM * BK; e += 256) {
-      const int r = e / BK, c = e % BK;
-      const int k = k0 + c;
+    const int k = k0 + sc;
+    const bool kin = k < ke;
+    int ky = 0, kx = 0, ch = 0;
+    if (implicit) {
+      const int tap = k / g.conv.C;
+      ch = k - tap * g.conv.C;
+      ky = tap / 3;
+      kx = tap - ky * 3;
+    }
+#pragma unroll
+    for (int i = 0; i < kRowsPerThread; ++i) {
+      const int r = sr + kStageRows * i;
       const int m = m0 + r, n = n0 + r;
-      As[r][c] = (m < g.M && k < ke) ? (g.a != nullptr ? g.a[m * g.sam + k * g.sak] : conv_element(g.conv, m, k))
-                                     : static_cast<uint16_t>(0);
-      Bs[r][c] = (n < g.N && k < ke) ? g.b[k * g.sbk + n * g.sbn] : static_cast<uint16_t>(0);
+      uint16_t av = 0;
+      if (kin) {
+        if (!implicit) {
+          if (m < g.M) av = g.a[m * g.sam + k * g.sak];
+        } else {
+          const int yi = oy[i] + ky, xi = ox[i] + kx;
+          if (yi >= 0 && yi < g.conv.H && xi >= 0 && xi < g.conv.W) {
+            if (g.conv.raw_prev != nullptr) {
+              const uint16_t* src = ch == 0 ? g.conv.raw_prev : g.conv.raw_cur;
+              av = raw_to_input(src[ob[i] * g.conv.raw_stride + yi * g.conv.W + xi], g.conv.inv_scale);
+            } else {
+              av = g.conv.x[((static_cast<long long>(ob[i]) * g.conv.H + yi) * g.conv.W + xi) * g.conv.C + ch];
+            }
+          }
+        }
+      }
+      As[r][sc] = av;
+      Bs[r][sc] = (n < g.N && kin) ? g.b[k * g.sbk + n * g.sbn] : static_cast<uint16_t>(0);
     }
     __syncthreads();
 #pragma unroll
@@ -187,6 +224,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(Gemm g) {
     __syncthreads();
   }
   // lane holds D[4 * lq + r][li] of each 16x16 block
+  const bool split = gridDim.z > 1;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -195,7 +233,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(Gemm g) {
       for (int r = 0; r < 4; ++r) {
         const int m = m0 + wm + i * 16 + lq * 4 + r, n = n0 + wn + j * 16 + li;
         if (m >= g.M || n >= g.N) continue;
-        if (g.partial != nullptr) {
+        if (split) {
           g.partial[(static_cast<size_t>(blockIdx.z) * g.M + m) * g.N + n] = acc[i][j][r];
         } else {
           gemm_store(g, m, n, acc[i][j][r]);
@@ -203,13 +241,133 @@ __global__ void __launch_bounds__(256) gemm_kernel(Gemm g) {
       }
 }
 
+// Sum of the split partials in split order 0, 1, ... (eight reads in
+// flight per thread), then the epilogue.
 __global__ void gemm_finish_kernel(Gemm g, int splits) {
   const long long total = static_cast<long long>(g.M) * g.N;
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= total) return;
+  const float* p = g.partial + i;
   float s = 0.0F;
-  for (int z = 0; z < splits; ++z) s += g.partial[z * total + i];
+  int z = 0;
+  for (; z + 8 <= splits; z += 8) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = p[(z + j) * total];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s += v[j];
+  }
+  for (; z < splits; ++z) s += p[z * total];
   gemm_store(g, static_cast<int>(i / g.N), static_cast<int>(i % g.N), s);
+}
+
+// ---------------------------------------------------------------------------
+// Small GEMM (few output tiles, K up to a few thousand: the layers of a pose
+// proposal, and the forward layers of small batches).  One workgroup per
+// 16 x 16 output tile; its kSmallWaves waves split the K steps (16 wide)
+// between them and load their operand fragments straight from global memory
+// into registers, kSmallChunk steps at a time, so that a layer costs one or
+// two memory round trips instead of one per K step.  Lane l holds
+// A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] (j = 0..3) of a
+// step and D[4 (l / 16) + r][l % 16] of the tile.  The per-wave tiles are
+// added in wave order through LDS, so the result does not depend on timing.
+constexpr int kSmallWaves = 8, kSmallChunk = 8;
+
+// A(m, k) of the im2col matrix for a row whose receptive-field origin is
+// (oy, ox) of pair ob (oy < -2 marks a row past M).
+__device__ inline uint16_t conv_a(const ConvView& v, int oy, int ox, int ob, int k) {
+  const int tap = k / v.C, ch = k - tap * v.C;
+  const int ky = tap / 3, kx = tap - ky * 3;
+  const int yi = oy + ky, xi = ox + kx;
+  if (yi < 0 || yi >= v.H || xi < 0 || xi >= v.W) return 0;
+  if (v.raw_prev != nullptr) {
+    const uint16_t* src = ch == 0 ? v.raw_prev : v.raw_cur;
+    return raw_to_input(src[ob * v.raw_stride + yi * v.W + xi], v.inv_scale);
+  }
+  return v.x[((static_cast<long long>(ob) * v.H + yi) * v.W + xi) * v.C + ch];
+}
+
+__device__ inline short as_short(uint16_t h) { return static_cast<short>(h); }
+
+__global__ void __launch_bounds__(kSmallWaves * 64) gemm_small_kernel(Gemm g, int vec_a, int vec_b) {
+  __shared__ float red[kSmallWaves][16][17];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lq = lane >> 4;
+  const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
+  const int m = m0 + li, n = n0 + li;  // this lane's A row and B column
+  const bool mok = m < g.M, nok = n < g.N;
+  const bool implicit = g.a == nullptr;
+  int oy = -4, ox = -4, ob = 0;
+  if (implicit && mok) {
+    const int xo = m % g.conv.Wo, t = m / g.conv.Wo;
+    oy = 2 * (t % g.conv.Ho) - 1;
+    ox = 2 * xo - 1;
+    ob = t / g.conv.Ho;
+  }
+  const int steps = (g.K + 15) / 16;
+  const int per = (steps + kSmallWaves - 1) / kSmallWaves;
+  const int s_begin = wave * per, s_end = min(steps, s_begin + per);
+  f32x4 acc = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
+  for (int c0 = s_begin; c0 < s_end; c0 += kSmallChunk) {
+    bf16x4 af[kSmallChunk], bfr[kSmallChunk];
+#pragma unroll
+    for (int c = 0; c < kSmallChunk; ++c) {
+      af[c] = bf16x4{0, 0, 0, 0};
+      bfr[c] = bf16x4{0, 0, 0, 0};
+      const int k = (c0 + c) * 16 + lq * 4;
+      if (c0 + c >= s_end || k >= g.K) continue;
+      const bool full = k + 3 < g.K;
+      if (mok) {
+        if (implicit) {
+          if (vec_a != 0 && full) {
+            // C % 4 == 0: the four k are channels ch..ch+3 of one tap
+            const int tap = k / g.conv.C, ch = k - tap * g.conv.C;
+            const int ky = tap / 3, kx = tap - ky * 3;
+            const int yi = oy + ky, xi = ox + kx;
+            if (yi >= 0 && yi < g.conv.H && xi >= 0 && xi < g.conv.W) {
+              af[c] = *reinterpret_cast<const bf16x4*>(
+                  g.conv.x + ((static_cast<long long>(ob) * g.conv.H + yi) * g.conv.W + xi) * g.conv.C + ch);
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+              if (k + j < g.K) af[c][j] = as_short(conv_a(g.conv, oy, ox, ob, k + j));
+          }
+        } else if (vec_a != 0 && full) {
+          af[c] = *reinterpret_cast<const bf16x4*>(g.a + m * g.sam + k);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (k + j < g.K) af[c][j] = as_short(g.a[m * g.sam + (k + j) * g.sak]);
+        }
+      }
+      if (nok) {
+        if (vec_b != 0 && full) {
+          bfr[c] = *reinterpret_cast<const bf16x4*>(g.b + n * g.sbn + k);
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (k + j < g.K) bfr[c][j] = as_short(g.b[(k + j) * g.sbk + n * g.sbn]);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kSmallChunk; ++c)
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[c], bfr[c], acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) red[wave][lq * 4 + r][li] = acc[r];
+  __syncthreads();
+  if (threadIdx.x < 256) {
+    const int row = threadIdx.x >> 4, col = threadIdx.x & 15;
+    const int om = m0 + row, on = n0 + col;
+    if (om < g.M && on < g.N) {
+      float s = 0.0F;
+#pragma unroll
+      for (int w = 0; w < kSmallWaves; ++w) s += red[w][row][col];
+      gemm_store(g, om, on, s);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -403,26 +561,36 @@ T* nalloc(size_t count) {
 void gemm(kf_posenet* net, hipStream_t s, Gemm g) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = tm * tn;
-  int splits = 1;
-  if (tiles < 256 && g.K > 1024) {  // only the long reductions (weight gradients, fc1) split
-    splits = std::min((g.K + BK - 1) / BK, (512 + tiles - 1) / tiles);
-  }
-  int kchunk = (g.K + splits - 1) / splits;
-  kchunk = (kchunk + BK - 1) / BK * BK;
-  splits = (g.K + kchunk - 1) / kchunk;
-  g.kchunk = kchunk;
-  if (splits > 1 && static_cast<size_t>(splits) * g.M * g.N > net->partial_floats) {
-    splits = std::max<int>(1, static_cast<int>(net->partial_floats / (static_cast<size_t>(g.M) * g.N)));
-    g.kchunk = ((g.K + splits - 1) / splits + BK - 1) / BK * BK;
-    splits = (g.K + g.kchunk - 1) / g.kchunk;
-  }
-  if (splits == 1) {
+  if (tiles <= 128 && g.K <= 4096) {
+    // vector fragment loads need four consecutive k in memory, 8-byte aligned
+    auto aligned8 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7U) == 0U; };
+    int vec_a = 0;
+    if (g.a == nullptr) {
+      vec_a = g.conv.raw_prev == nullptr && g.conv.C % 4 == 0 && aligned8(g.conv.x) ? 1 : 0;
+    } else {
+      vec_a = g.sak == 1 && g.sam % 4 == 0 && aligned8(g.a) ? 1 : 0;
+    }
+    const int vec_b = g.sbk == 1 && g.sbn % 4 == 0 && aligned8(g.b) ? 1 : 0;
     g.partial = nullptr;
-    g.kchunk = (g.K + BK - 1) / BK * BK;
-    hipLaunchKernelGGL(gemm_kernel, dim3(tm, tn, 1), dim3(256), 0, s, g);
-  } else {
-    g.partial = net->partial;
-    hipLaunchKernelGGL(gemm_kernel, dim3(tm, tn, splits), dim3(256), 0, s, g);
+    hipLaunchKernelGGL(gemm_small_kernel, dim3((g.M + 15) / 16, (g.N + 15) / 16), dim3(kSmallWaves * 64), 0, s, g,
+                       vec_a, vec_b);
+    KF_HIP(hipGetLastError());
+    return;
+  }
+  // Few tiles and a long reduction: split K until about 512 workgroups run,
+  // each over as little as one K step (the loop is latency bound at these
+  // sizes: one pose proposal is a handful of tiles per layer).
+  int splits = 1;
+  if (tiles <= kMaxTiles && g.K > 2 * BK) {
+    splits = std::min((g.K + BK - 1) / BK, (512 + tiles - 1) / tiles);
+    const size_t cap = net->partial_floats / (static_cast<size_t>(g.M) * g.N);
+    splits = static_cast<int>(std::min<size_t>(static_cast<size_t>(splits), std::max<size_t>(cap, 1)));
+  }
+  g.kchunk = ((g.K + splits - 1) / splits + BK - 1) / BK * BK;
+  splits = (g.K + g.kchunk - 1) / g.kchunk;
+  g.partial = splits > 1 ? net->partial : nullptr;
+  hipLaunchKernelGGL(gemm_kernel, dim3(tm, tn, splits), dim3(256), 0, s, g);
+  if (splits > 1) {
     const long long total = static_cast<long long>(g.M) * g.N;
     hipLaunchKernelGGL(gemm_finish_kernel, dim3(blocks_for(total)), dim3(256), 0, s, g, splits);
   }

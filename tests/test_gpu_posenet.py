@@ -125,11 +125,11 @@ def test_device_renderer_matches_host(K, torch_gpu):
 
 def test_network_learns_motion(K, PN, torch_gpu):
     cfg = pipeline_config(K)
-    pairs, targets, stats = PN.generate_pairs(cfg, 6144, labels="truth", seed=1)
-    test_pairs, test_targets, _ = PN.generate_pairs(cfg, 256, labels="truth", seed=101, scene_seeds=(1009,))
+    pairs, targets, stats = PN.generate_pairs(cfg, 8192, labels="truth", seed=1)
+    test_pairs, test_targets, _ = PN.generate_pairs(cfg, 512, labels="truth", seed=101, scene_seeds=(1009, 1010))
     net = PN.PoseNet(PN.PoseNetConfig(seed=1))
     before = PN.proposal_errors(net, test_pairs, test_targets)
-    hist = PN.train(net, pairs, targets, epochs=10, seed=1)
+    hist = PN.train(net, pairs, targets, epochs=12, seed=1)
     after = PN.proposal_errors(net, test_pairs, test_targets)
     print(f"loss {hist}; held-out proposal error before {before}, after {after}")
     assert hist[-1] < 0.5 * hist[0]
