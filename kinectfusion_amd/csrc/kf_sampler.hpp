@@ -21,6 +21,26 @@ struct SlotCache {
   uint32_t slot = kNoSlot;
 };
 
+// Two x-neighbouring voxels as one vector load (x-fastest layout: they are
+// adjacent in memory; the pair is only as aligned as one voxel).
+typedef uint32_t u32x2_a4 __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f32x4_a8 __attribute__((ext_vector_type(4), aligned(8)));
+
+template <typename Raw>
+__device__ inline void load_pair(const Raw* p, Raw& a, Raw& b);
+template <>
+__device__ inline void load_pair<uint32_t>(const uint32_t* p, uint32_t& a, uint32_t& b) {
+  const u32x2_a4 v = *reinterpret_cast<const u32x2_a4*>(p);
+  a = v.x;
+  b = v.y;
+}
+template <>
+__device__ inline void load_pair<float2>(const float2* p, float2& a, float2& b) {
+  const f32x4_a8 v = *reinterpret_cast<const f32x4_a8*>(p);
+  a = make_float2(v.x, v.y);
+  b = make_float2(v.z, v.w);
+}
+
 template <int Kind>
 struct DenseStore {
   using Raw = typename VoxelIO<Kind>::Raw;
@@ -28,6 +48,10 @@ struct DenseStore {
   const Raw* __restrict__ vox;
   const float4* __restrict__ colors;
   __device__ inline Raw voxel(const Grid& g, int x, int y, int z) const { return vox[g.flat(x, y, z)]; }
+  // voxels (x, y, z) and (x + 1, y, z), both inside the grid
+  __device__ inline void pair(const Grid& g, int x, int y, int z, Raw& a, Raw& b) const {
+    load_pair<Raw>(vox + g.flat(x, y, z), a, b);
+  }
   __device__ inline Raw voxel_cached(const Grid& g, int x, int y, int z, SlotCache&) const {
     return vox[g.flat(x, y, z)];
   }
@@ -113,6 +137,14 @@ struct Sampler {
   // Loads of the eight corners (z -> y -> x order); bit c of `inb` marks a
   // corner inside the grid.
   __device__ inline void load8(const Stencil& st, typename IO::Raw (&raw)[8], unsigned& inb, SlotCache& cache) const {
+    if constexpr (Store::kDirect) {
+      if (st.ok && st.interior) {  // four x-pairs
+#pragma unroll
+        for (int c = 0; c < 8; c += 2) s.pair(g, st.bx, st.by + ((c >> 1) & 1), st.bz + (c >> 2), raw[c], raw[c + 1]);
+        inb = 0xFFU;
+        return;
+      }
+    }
     inb = 0U;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -152,6 +184,17 @@ struct Sampler {
 
   // TrilinearSampler::tsdf + blend_tsdf.
   __device__ bool tsdf(V3 p, bool require_all, float& out) const {
+    if constexpr (Store::kDirect) {
+      // the eight corners as four x-pair loads; blend() is the same float
+      // sequence as the loop below
+      const Stencil st = stencil(p);
+      if (!st.ok) return false;
+      typename IO::Raw raw[8];
+      unsigned inb;
+      SlotCache cache;
+      load8(st, raw, inb, cache);
+      return blend(st, raw, inb, require_all, out);
+    }
     const V3 gc = grid_coords(p);
     if (!contains(gc)) return false;
     const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
