@@ -52,6 +52,17 @@ struct DenseStore {
   __device__ inline void pair(const Grid& g, int x, int y, int z, Raw& a, Raw& b) const {
     load_pair<Raw>(vox + g.flat(x, y, z), a, b);
   }
+  // The eight corners of the cell at (x, y, z) (z -> y -> x order), all
+  // inside the grid: one 64-bit index, then four x-pair loads at fixed
+  // row and slice offsets.
+  __device__ inline void cell(const Grid& g, int x, int y, int z, Raw (&raw)[8]) const {
+    const Raw* p = vox + g.flat(x, y, z);
+    const size_t row = static_cast<size_t>(g.nx), slice = static_cast<size_t>(g.nx) * static_cast<size_t>(g.ny);
+    load_pair<Raw>(p, raw[0], raw[1]);
+    load_pair<Raw>(p + row, raw[2], raw[3]);
+    load_pair<Raw>(p + slice, raw[4], raw[5]);
+    load_pair<Raw>(p + slice + row, raw[6], raw[7]);
+  }
   __device__ inline Raw voxel_cached(const Grid& g, int x, int y, int z, SlotCache&) const {
     return vox[g.flat(x, y, z)];
   }
@@ -139,8 +150,7 @@ struct Sampler {
   __device__ inline void load8(const Stencil& st, typename IO::Raw (&raw)[8], unsigned& inb, SlotCache& cache) const {
     if constexpr (Store::kDirect) {
       if (st.ok && st.interior) {  // four x-pairs
-#pragma unroll
-        for (int c = 0; c < 8; c += 2) s.pair(g, st.bx, st.by + ((c >> 1) & 1), st.bz + (c >> 2), raw[c], raw[c + 1]);
+        s.cell(g, st.bx, st.by, st.bz, raw);
         inb = 0xFFU;
         return;
       }
@@ -182,14 +192,45 @@ struct Sampler {
     return true;
   }
 
+  // blend() of a cell whose eight corners all lie inside the grid, without
+  // branches: a corner that is unobserved or not finite adds +0 to both sums,
+  // which leaves them unchanged (neither sum is ever -0: they start at +0 and
+  // a round-to-nearest sum is -0 only when both terms are).
+  __device__ inline bool blend_all(const Stencil& st, const typename IO::Raw (&raw)[8], bool require_all,
+                                   float& out) const {
+    float acc = 0.0F, wsum = 0.0F;
+    int valid = 0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      float t, w;
+      bool obs, fin;
+      IO::unpack(raw[c], t, w, obs, fin);
+      const bool use = obs && fin;
+      const float cw = axis_w(st.fr.x, c & 1) * axis_w(st.fr.y, (c >> 1) & 1) * axis_w(st.fr.z, c >> 2);
+      acc += use ? cw * t : 0.0F;
+      wsum += use ? cw : 0.0F;
+      valid += use ? 1 : 0;
+    }
+    if (require_all && valid < 8) return false;
+    if (wsum <= 1.0e-6F) return false;
+    const float d = acc / wsum;
+    if (!isfinite(d)) return false;
+    out = d;
+    return true;
+  }
+
   // TrilinearSampler::tsdf + blend_tsdf.
   __device__ bool tsdf(V3 p, bool require_all, float& out) const {
     if constexpr (Store::kDirect) {
-      // the eight corners as four x-pair loads; blend() is the same float
-      // sequence as the loop below
+      // the eight corners as four x-pair loads; blend_all() and blend() are
+      // the same float sequence as the loop below
       const Stencil st = stencil(p);
       if (!st.ok) return false;
       typename IO::Raw raw[8];
+      if (st.interior) {
+        s.cell(g, st.bx, st.by, st.bz, raw);
+        return blend_all(st, raw, require_all, out);
+      }
       unsigned inb;
       SlotCache cache;
       load8(st, raw, inb, cache);
