@@ -26,19 +26,6 @@ __device__ inline uint32_t pixel_from_color(V3 c) {
   return to_byte(c.x) | (to_byte(c.y) << 8U) | (to_byte(c.z) << 16U) | (0xFFU << 24U);
 }
 
-// Exit parameter (ray length) of the ray gc(len) = a + b len from the box
-// [lo, hi) along each axis, in FP64.
-__device__ __forceinline__ double box_exit(const double (&a)[3], const double (&b)[3], const double (&lo)[3],
-                                           const double (&hi)[3]) {
-  double t = 1.0e300;
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    if (b[k] > 1.0e-12) t = fmin(t, (hi[k] - a[k]) / b[k]);
-    else if (b[k] < -1.0e-12) t = fmin(t, (lo[k] - a[k]) / b[k]);
-  }
-  return t;
-}
-
 template <int Kind, typename Store>
 __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, RayParams rp, Rigid pose_arg,
                                                        IcpState* state, float4* __restrict__ pts,
@@ -151,8 +138,13 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
           }
           bool fr[kAhead];
           fr[0] = true;
+          // unconditional loads (a sample out of range reads word 0), so that
+          // all of them are in flight before the first is needed
+          uint32_t fw[kAhead];
 #pragma unroll
-          for (int i = 1; i < kAhead; ++i) fr[i] = fbs[i] >= 0 && bm.bit(free_bits, fbs[i]);
+          for (int i = 1; i < kAhead; ++i) fw[i] = free_bits[fbs[i] >= 0 ? fbs[i] >> 5 : 0];
+#pragma unroll
+          for (int i = 1; i < kAhead; ++i) fr[i] = fbs[i] >= 0 && ((fw[i] >> (fbs[i] & 31)) & 1U) != 0U;
           float last = ls[0], next = ls[1];
           int last_fb = fb;
           bool run = true;

@@ -131,9 +131,10 @@ struct Sampler {
     bool ok;        // the point lies inside the grid
     bool interior;  // all eight corners lie inside the grid
   };
-  __device__ inline Stencil stencil(V3 p) const {
+  __device__ inline Stencil stencil(V3 p) const { return stencil_gc(grid_coords(p)); }
+  // the same from grid coordinates already computed
+  __device__ inline Stencil stencil_gc(V3 gc) const {
     Stencil st;
-    const V3 gc = grid_coords(p);
     st.ok = contains(gc);
     const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
     const V3 fl = mk3(floorf(gg.x), floorf(gg.y), floorf(gg.z));
