@@ -233,7 +233,8 @@ __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
   base = __shfl(base, 0);
   if (keep) {
     const unsigned long long below = mask & ((1ULL << lane) - 1ULL);
-    list[base + __popcll(below)] = b;
+    list[base + __popcll(below)] = pack_brick(static_cast<int>(b % bnx), static_cast<int>((b / bnx) % bny),
+                                              static_cast<int>(b / (bnx * bny)));
   }
 }
 
@@ -397,28 +398,30 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
   }
 }
 
-// The voxel updates of one (x, y) column of a brick.
+// The voxel updates of one z-part (4 voxels from z0) of an (x, y) column.
 template <int Kind, bool Color, bool Count>
 __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, const Rigid& w2c,
                                         const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
                                         typename VoxelIO<Kind>::Raw* __restrict__ vox,
-                                        float4* __restrict__ colors, int x, int y, int bz,
+                                        float4* __restrict__ colors, int x, int y, int z0,
                                         unsigned int& my_fused, unsigned int& my_color, bool& ones_ok,
                                         bool& complete) {
-  int xs[kBrickZ], ys[kBrickZ], zs[kBrickZ];
-  size_t vis[kBrickZ];
-  bool inside[kBrickZ];
+  constexpr int kHalf = kBrickZ / 2;  // one z-part of a block
+  int xs[kHalf], ys[kHalf], zs[kHalf];
+  size_t vis[kHalf];
+  bool inside[kHalf];
+  const size_t v0 = g.flat(x, y, z0), slice = static_cast<size_t>(g.nx) * static_cast<size_t>(g.ny);
 #pragma unroll
-  for (int k = 0; k < kBrickZ; ++k) {
+  for (int k = 0; k < kHalf; ++k) {
     xs[k] = x;
     ys[k] = y;
-    zs[k] = bz * kBrickZ + k;
+    zs[k] = z0 + k;
     inside[k] = zs[k] < g.nz;
-    vis[k] = inside[k] ? g.flat(x, y, zs[k]) : 0;
+    vis[k] = inside[k] ? v0 + k * slice : 0;
   }
   bool any = false;
-  fuse_batch<Kind, Color, Count, kBrickZ>(g, f, w2c, pix, color_img, vox, colors, xs, ys, zs, vis, inside, my_fused,
-                                          my_color, any, ones_ok, complete);
+  fuse_batch<Kind, Color, Count, kHalf>(g, f, w2c, pix, color_img, vox, colors, xs, ys, zs, vis, inside, my_fused,
+                                        my_color, any, ones_ok, complete);
   return any;
 }
 
@@ -440,60 +443,62 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
                                                          unsigned long long* __restrict__ fused,
                                                          uint32_t* __restrict__ obs_bits, BlockMap bm,
                                                          uint32_t* __restrict__ part_bits) {
-  __shared__ uint32_t parts_ok, parts_complete;
+  // A brick is 8 blocks along x of one y-half; a thread's column of 8
+  // voxels is both z-halves of its block.  parts_*[h] bit j: block j's part
+  // (y-half, z-half h) had every voxel ok / complete in every row.
+  __shared__ uint32_t parts_ok[2], parts_complete[2];
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
-  const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
   const uint32_t n = *count;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   unsigned int my_fused = 0, my_color = 0;
   for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
     const uint32_t b = list[li];
-    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
-    const int x = bx * kBrickX + tx, y = by * kBrickY + ty;
-    bool any = false, ones_ok = false, complete = false;
-    if (threadIdx.x == 0) {
-      parts_ok = 0xFFU;
-      parts_complete = 0xFFU;
+    const int bx = static_cast<int>(b & 0xFFU), by = static_cast<int>((b >> 8U) & 0xFFFU),
+              bz = static_cast<int>(b >> 20U);
+    const int x = bx * kBrickX + tx, y = by * kBrickY + ty, z0 = bz * kBrickZ;
+    if (threadIdx.x < 2) {
+      parts_ok[threadIdx.x] = 0xFFU;
+      parts_complete[threadIdx.x] = 0xFFU;
     }
     __syncthreads();
-    // The brick's rows of 64 voxels span 8 occupancy blocks along x and one
-    // along y and z: one ballot per wave marks them (lane 0 publishes).
-    if (x < g.nx && y < g.ny) any = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors,
-                                                                           x, y, bz, my_fused, my_color, ones_ok,
-                                                                           complete);
-    const unsigned long long om = __ballot(ones_ok), cm = __ballot(complete);
-    if (tx == 0) {
-      uint32_t ok = 0U, cpl = 0U;
-      for (int j = 0; j < 8; ++j) {
-        ok |= ((om >> (8 * j)) & 0xFFULL) == 0xFFULL ? (1U << j) : 0U;
-        cpl |= ((cm >> (8 * j)) & 0xFFULL) == 0xFFULL ? (1U << j) : 0U;
-      }
-      atomicAnd(&parts_ok, ok);
-      atomicAnd(&parts_complete, cpl);
+    bool any0 = false, any1 = false, ok0 = false, ok1 = false, cpl0 = false, cpl1 = false;
+    if (x < g.nx && y < g.ny) {
+      any0 = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors, x, y, z0, my_fused,
+                                                  my_color, ok0, cpl0);
+      any1 = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors, x, y, z0 + kBrickZ / 2,
+                                                  my_fused, my_color, ok1, cpl1);
     }
-    const unsigned long long mask = __ballot(any);
-    if (tx == 0 && mask != 0ULL) {
-      const int blk_y = (by * kBrickY) >> 3, blk_z = (bz * kBrickZ) >> 3;
-      for (int j = 0; j < 8; ++j) {
-        const int blk_x = bx * 8 + j;
-        if (((mask >> (8 * j)) & 0xFFULL) == 0ULL || blk_x >= bm.bx) continue;
-        const int fb = bm.flat(blk_x, blk_y, blk_z);
+    // Lanes 8j..8j+7 of a wave are block j's row: one ballot per flag; lane
+    // j < 8 folds its block's byte of each.
+    const unsigned long long om0 = __ballot(ok0), cm0 = __ballot(cpl0), om1 = __ballot(ok1),
+                             cm1 = __ballot(cpl1), am = __ballot(any0 || any1);
+    if (tx < 8) {
+      const int j = tx;
+      auto full = [&](unsigned long long m) { return ((m >> (8 * j)) & 0xFFULL) == 0xFFULL; };
+      const uint32_t bit = 1U << j;
+      if (!full(om0)) atomicAnd(&parts_ok[0], ~bit);
+      if (!full(cm0)) atomicAnd(&parts_complete[0], ~bit);
+      if (!full(om1)) atomicAnd(&parts_ok[1], ~bit);
+      if (!full(cm1)) atomicAnd(&parts_complete[1], ~bit);
+      const int blk_x = bx * 8 + j;
+      if (((am >> (8 * j)) & 0xFFULL) != 0ULL && blk_x < bm.bx) {
+        const int fb = bm.flat(blk_x, (by * kBrickY) >> 3, z0 >> 3);
         atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
       }
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      // this brick is the (y-half, z-half) part of 8 blocks along x
-      const int blk_y = (by * kBrickY) >> 3, blk_z = (bz * kBrickZ) >> 3;
-      const int part = (((by * kBrickY) >> 2) & 1) * 2 + (((bz * kBrickZ) >> 2) & 1);
-      for (int j = 0; j < 8; ++j) {
-        const int blk_x = bx * 8 + j;
-        if (blk_x >= bm.bx) break;
-        const bool ok = ((parts_ok >> j) & 1U) != 0U, cpl = ((parts_complete >> j) & 1U) != 0U;
-        if (!ok || cpl) set_part(part_bits, bm.flat(blk_x, blk_y, blk_z), part, ok);  // else unchanged
+    if (threadIdx.x < 16) {
+      // part (y-half, z-half h) of block j
+      const int j = threadIdx.x & 7, h = threadIdx.x >> 3;
+      const int blk_x = bx * 8 + j;
+      if (blk_x < bm.bx) {
+        const int part = (((by * kBrickY) >> 2) & 1) * 2 + h;
+        const bool ok = ((parts_ok[h] >> j) & 1U) != 0U, cpl = ((parts_complete[h] >> j) & 1U) != 0U;
+        if (!ok || cpl) set_part(part_bits, bm.flat(blk_x, (by * kBrickY) >> 3, z0 >> 3), part, ok);
       }
     }
+    __syncthreads();
   }
   if (Count) {
     for (int off = 32; off > 0; off >>= 1) {
