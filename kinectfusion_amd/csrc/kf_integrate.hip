@@ -275,7 +275,7 @@ __device__ inline bool same_raw(Raw a, Raw b) {
 //  * sd < -truncation: rejected.
 // A voxel whose stored bits do not change (saturated free space) is not
 // written back.
-template <int Kind, bool Color, bool Count, int N>
+template <int Kind, bool Color, bool Count, int N, bool Column = false>
 __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rigid& w2c,
                                   const float4* __restrict__ pix, const uint32_t* __restrict__ color_img,
                                   typename VoxelIO<Kind>::Raw* __restrict__ vox, float4* __restrict__ colors,
@@ -285,12 +285,29 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
   using IO = VoxelIO<Kind>;
   int pi[N];
   V3 cp[N];
+  // A column (every voxel at the same x, y): the voxel centre's x and y and
+  // each row's first two products and their sum are the same for all of
+  // them, so they are formed once; the rest of xform(w2c, centre) is the
+  // same float sequence: ((r.x cx + r.y cy) + r.z cz) + t.
+  float a0 = 0.0F, a1 = 0.0F, a2 = 0.0F;
+  if constexpr (Column) {
+    const float cx = g.ox + ((static_cast<float>(xs[0]) + 0.5F) * g.voxel);
+    const float cy = g.oy + ((static_cast<float>(ys[0]) + 0.5F) * g.voxel);
+    a0 = (w2c.r0.x * cx) + (w2c.r0.y * cy);
+    a1 = (w2c.r1.x * cx) + (w2c.r1.y * cy);
+    a2 = (w2c.r2.x * cx) + (w2c.r2.y * cy);
+  }
 #pragma unroll
   for (int k = 0; k < N; ++k) {
     pi[k] = -1;
     cp[k] = mk3(0.0F, 0.0F, 0.0F);
     if (!inside[k]) continue;
-    cp[k] = xform(w2c, g.center(xs[k], ys[k], zs[k]));
+    if constexpr (Column) {
+      const float cz = g.oz + ((static_cast<float>(zs[k]) + 0.5F) * g.voxel);
+      cp[k] = mk3((a0 + (w2c.r0.z * cz)) + w2c.t.x, (a1 + (w2c.r1.z * cz)) + w2c.t.y, (a2 + (w2c.r2.z * cz)) + w2c.t.z);
+    } else {
+      cp[k] = xform(w2c, g.center(xs[k], ys[k], zs[k]));
+    }
     if (cp[k].z <= 0.0F) continue;
     const float rz = __builtin_amdgcn_rcpf(cp[k].z);
     const float qx = (f.k.fx * cp[k].x) * rz, qy = (f.k.fy * cp[k].y) * rz;
@@ -319,35 +336,38 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
   // sd), 3 no fuse at an in-image pixel without depth: the voxel is read
   // only to learn whether it already holds an observed 1 (a depth dropout
   // inside free space then does not break the part's completeness).
+  // The decisions are selects; only voxels whose approximate distance lies
+  // within a margin of the band take the exact (IEEE) path.
   int mode[N];
   float sd[N];
+  bool exact[N];
 #pragma unroll
   for (int k = 0; k < N; ++k) {
-    mode[k] = 0;
-    sd[k] = 0.0F;
-    if (p[k].x < 0.0F || p[k].z <= 0.0F) {  // no depth, or w <= 0 drops the fuse
-      if (pi[k] >= 0) mode[k] = 3;
-      continue;
-    }
+    const bool has_depth = !(p[k].x < 0.0F || p[k].z <= 0.0F);  // no depth, or w <= 0 drops the fuse
+    int m = (!has_depth && pi[k] >= 0) ? 3 : 0;
+    float s = 0.0F;
+    bool ex = false;
     if (f.projective) {
       const float sa = __builtin_amdgcn_sqrtf(dot3(cp[k], cp[k])) * __builtin_amdgcn_rcpf(p[k].y);
       const float sda = p[k].x - sa;
       const float margin = 1.0e-6F + 4.0e-6F * sa;
-      if (sda < -p[k].w - margin) continue;
-      if (sda >= p[k].w + margin) {
-        mode[k] = 1;
-        continue;
-      }
-      sd[k] = p[k].x - (norm3(cp[k]) / p[k].y);
+      const bool fr = sda >= p[k].w + margin;
+      const bool rj = sda < -p[k].w - margin;
+      m = has_depth ? (fr ? 1 : (rj ? 0 : 2)) : m;
+      ex = has_depth && !fr && !rj;
     } else {
-      sd[k] = p[k].x - cp[k].z;
-      if (sd[k] >= p[k].w) {
-        mode[k] = 1;
-        continue;
-      }
+      s = p[k].x - cp[k].z;
+      m = has_depth ? (s >= p[k].w ? 1 : (s < -p[k].w ? 0 : 2)) : m;
     }
-    if (sd[k] < -p[k].w) continue;
-    mode[k] = 2;
+    mode[k] = m;
+    sd[k] = s;
+    exact[k] = ex;
+  }
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    if (!exact[k]) continue;
+    sd[k] = p[k].x - (norm3(cp[k]) / p[k].y);
+    if (sd[k] < -p[k].w) mode[k] = 0;
   }
   typename IO::Raw raw[N];
 #pragma unroll
@@ -420,8 +440,8 @@ __device__ inline bool integrate_column(const Grid& g, const FrameParams& f, con
     vis[k] = inside[k] ? v0 + k * slice : 0;
   }
   bool any = false;
-  fuse_batch<Kind, Color, Count, kHalf>(g, f, w2c, pix, color_img, vox, colors, xs, ys, zs, vis, inside, my_fused,
-                                        my_color, any, ones_ok, complete);
+  fuse_batch<Kind, Color, Count, kHalf, true>(g, f, w2c, pix, color_img, vox, colors, xs, ys, zs, vis, inside,
+                                              my_fused, my_color, any, ones_ok, complete);
   return any;
 }
 
