@@ -541,7 +541,7 @@ __device__ void sym_eig6(const double* eq, double* ev) {
 }
 
 // Prepare a level: status cleared, model pose = running pose.
-__global__ void icp_begin_level_kernel(IcpState* st, int level) {
+__global__ void __launch_bounds__(64) icp_begin_level_kernel(IcpState* st, int level) {
   if (threadIdx.x != 0) return;
   kf_icp_result& r = st->results[level];
   r = kf_icp_result{};
@@ -564,7 +564,7 @@ __global__ void icp_begin_level_kernel(IcpState* st, int level) {
 }
 
 // Stability check and result record (icp_optimizer.cpp:108-169).
-__global__ void icp_finish_level_kernel(IcpState* st, DeviceIcpParams p, int level) {
+__global__ void __launch_bounds__(64) icp_finish_level_kernel(IcpState* st, DeviceIcpParams p, int level) {
   if (threadIdx.x != 0) return;
   if (st->frame_failed) return;
   kf_icp_result& r = st->results[level];
