@@ -328,7 +328,8 @@ void record(kf_context* ctx, int slot) {
 // Coarse-to-fine tracking from the device pose (track_pose).
 // Coarse-to-fine tracking with per-level iteration counts (0 skips a level
 // and its raycast).
-void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing, const int32_t* iterations) {
+void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing, const int32_t* iterations,
+                  bool single_launch = false) {
   const kf_config& c = ctx->cfg;
   int slot = 2;
   for (int l = c.levels - 1; l >= 0; --l) {
@@ -340,7 +341,7 @@ void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing, const 
     if (timing) record(ctx, slot++);
     if (iterations[l] > 0) {
       launch_icp_level(ctx, L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k, iterations[l], p,
-                       l);
+                       l, single_launch);
     }
     if (timing) record(ctx, slot++);
   }
@@ -388,7 +389,8 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     while (c.learned_iterations[finest] == 0) ++finest;
     hipLaunchKernelGGL(learned_check_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, finest,
                        c.learned_max_rms);
-    track_levels(ctx, p, false, c.icp_iterations);
+    // the fallback pass runs only when the check failed: one launch per trial
+    track_levels(ctx, p, false, c.icp_iterations, true);
     hipLaunchKernelGGL(learned_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     record(ctx, 2 * c.levels + 1);  // the fallback run counts as tracking time
   } else {
@@ -584,6 +586,8 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->fused = dalloc<unsigned long long>(4);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     ctx->partials = dalloc<float>(512 * 32);
+    ctx->ticket = dalloc<unsigned int>(1);
+    KF_HIP(hipMemsetAsync(ctx->ticket, 0, sizeof(unsigned int), ctx->stream));
     ctx->icp = dalloc<IcpState>(1);
     KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
     KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));
@@ -639,6 +643,7 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->counters);
   dfree(ctx->fused);
   dfree(ctx->partials);
+  dfree(ctx->ticket);
   dfree(ctx->icp);
   dfree(ctx->trajectory);
   dfree(ctx->prev_coarse);

@@ -464,6 +464,51 @@ __global__ void __launch_bounds__(256) icp_trial_sweep_kernel(Maps m, DeviceIcpP
   sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
 }
 
+// Ticket protocol: returns true in the workgroup that finished last, after
+// which the partial rows of every workgroup are visible (device-scope
+// release before the ticket, acquire after it).
+__device__ inline bool last_block(unsigned int* ticket) {
+  __shared__ bool is_last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) is_last = atomicAdd(ticket, 1U) == gridDim.x - 1U;
+  __syncthreads();
+  if (is_last) __threadfence();
+  return is_last;
+}
+
+// One trial in a single launch: the sweep, then in the last workgroup to
+// finish the reduction and the FP64 solve.  The fences cost more than the
+// extra launch of the two-kernel form when the trial runs, so this form
+// serves passes that usually exit at once (latched status).
+__global__ void __launch_bounds__(256) icp_trial_single_kernel(Maps m, DeviceIcpParams p, IcpState* st,
+                                                                float* __restrict__ partial,
+                                                                unsigned int* __restrict__ ticket) {
+  __shared__ float lds[4][kSums + 1];
+  __shared__ float combined[kSums + 1];
+  __shared__ float xf[24];
+  if (st->status != 0 || st->frame_failed) return;  // latched: nothing to do
+  if (threadIdx.x == 0) {
+    double inv[16];
+    rigid_inverse64(st->model_pose, inv);
+    for (int i = 0; i < 12; ++i) {
+      xf[i] = static_cast<float>(st->pose[i]);
+      xf[12 + i] = static_cast<float>(inv[i]);
+    }
+  }
+  __syncthreads();
+  const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
+  sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
+  if (!last_block(ticket)) return;
+  combine_partials(partial, gridDim.x, combined);
+  __syncthreads();
+  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
+  if (threadIdx.x == 0) {
+    *ticket = 0U;
+    run_trial(st, p, combined);
+  }
+}
+
 // The trial's reduction (one workgroup: the partial rows into st->sums32)
 // and its FP64 solve and pose update (one thread).  A kernel of its own, so
 // the partial rows of every sweep workgroup are visible without a
@@ -632,10 +677,15 @@ void launch_icp_begin_level(kf_context* ctx, int level) {
 
 void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                       const float4* md_v, const float4* md_n, int mw, int mh, Intr k, int iterations,
-                      const DeviceIcpParams& p, int level) {
+                      const DeviceIcpParams& p, int level, bool single_launch) {
   const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
   const int nb = blocks_for(lw * lh);
   for (int it = 0; it < iterations; ++it) {
+    if (single_launch) {
+      hipLaunchKernelGGL(icp_trial_single_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
+                         ctx->partials, ctx->ticket);
+      continue;
+    }
     hipLaunchKernelGGL(icp_trial_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
                        ctx->partials);
     hipLaunchKernelGGL(icp_trial_solve_kernel, dim3(1), dim3(256), 0, ctx->stream, p, ctx->icp, ctx->partials, nb);

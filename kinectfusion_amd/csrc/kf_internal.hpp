@@ -353,6 +353,7 @@ struct kf_context {
   unsigned long long* fused = nullptr;
   // icp scratch
   float* partials = nullptr;      // per-block partial sums (32 floats each)
+  unsigned int* ticket = nullptr; // last-workgroup counter of the single-launch trial (zero between launches)
   kf::IcpState* icp = nullptr;    // device
   kf::IcpState* icp_host = nullptr;  // pinned mirror for results
   double* trajectory = nullptr;   // device, 16 doubles per frame
@@ -401,9 +402,13 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, 
                       bool accumulate = false);
 void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
                     bool skip_if_failed);
+// single_launch: one kernel per trial (sweep, then the reduction and the
+// solve in the last workgroup behind device-scope fences) -- for passes that
+// rarely run and whose launches are mostly early exits (the learned warm
+// start's fallback); otherwise two kernels per trial.
 void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                       const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
-                      int iterations, const DeviceIcpParams& p, int level_index);
+                      int iterations, const DeviceIcpParams& p, int level_index, bool single_launch = false);
 void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                             const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
                             const float* camera12, const float* model12, float max_dist,

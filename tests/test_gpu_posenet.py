@@ -172,3 +172,29 @@ def test_learned_warm_start_tracks_and_gains(K, PN, torch_gpu):
     assert max(gt_err) < 0.01
     p.close()
     net.close()
+
+
+def test_fallback_pass_equals_constant_velocity_tracking(K, PN, torch_gpu):
+    """With an unreachable residual bar every learned proposal is rejected:
+    the fallback pass (one launch per trial) then tracks from the
+    constant-velocity seed with the full schedule, and the trajectory is the
+    constant-velocity pipeline's own, bit for bit (the same partial rows,
+    reduction order and FP64 solve)."""
+    net = PN.PoseNet(PN.PoseNetConfig(seed=3))
+    scene = K.SyntheticScene("room", seed=0)
+    frames = [scene.frame(i * 2) for i in range(6)]
+    trajs = []
+    for init in ("learned", "constant-velocity"):
+        cfg = pipeline_config(K, voxel=0.02, init=init, learned_iterations=(4, 0, 0), learned_max_rms=1e-12)
+        p = K.Pipeline(cfg)
+        if init == "learned":
+            p.attach_posenet(net)
+        for i, (depth, rgba, _) in enumerate(frames):
+            res = p.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba))
+            assert res.tracked == 1
+            if init == "learned" and i > 0:
+                assert res.used_learned == 0
+        trajs.append(p.trajectory())
+        p.close()
+    net.close()
+    assert np.array_equal(trajs[0], trajs[1])
