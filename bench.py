@@ -188,6 +188,19 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
+def committed_traffic(voxel: str):
+    """HBM bytes per launch of the integrate kernel as measured by two rocprofv3 counter passes
+    (FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter reads for a plain 256 MiB copy),
+    from the committed summary under profiles/; None when there is none for this voxel type."""
+    path = Path(__file__).resolve().parent / "profiles" / f"integrate_kernel_512cubed_{voxel}_pmc.md"
+    if not path.exists():
+        return None, None
+    for line in path.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1]), f"profiles/{path.name}"
+    return None, None
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +278,8 @@ def main() -> int:
     stage_ms = {k: round(float(np.mean(times[:, j])), 4) for j, k in
                 enumerate(["pyramid", "proposal", "raycast", "icp", "integrate", "integrate_kernel", "total"])}
 
+    traffic, traffic_src = committed_traffic(voxel)
+
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle on {args.cpu_frames} tracked frames (single thread)")
@@ -294,9 +309,10 @@ def main() -> int:
         "ate_m": round(r["ate_m"], 6),
         "stage_ms": stage_ms,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
                      "kernel": f"integrate_kernel<{voxel}, colour>",
-                     "bytes_per_launch": float(alg_bytes.mean()), "avg_launch_ms": float(kern_ms.mean())},
+                     "bytes_per_launch": float(alg_bytes.mean()), "avg_launch_ms": float(kern_ms.mean()),
+                     "traffic_source": traffic_src},
         "cpu_baseline": cpu,
     }
     if init == "learned":
