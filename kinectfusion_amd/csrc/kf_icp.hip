@@ -359,11 +359,17 @@ __device__ void compose_increment(const double* x, double* pose) {
 __device__ void sym_eig6(const double* eq, double* ev);
 
 // check_system_stability (icp_optimizer.cpp:319-347) on FP64 sums.
+__device__ bool stable_from_eigenvalues(const double* ev, const DeviceIcpParams& p, double* min_eig, double* cond);
+
 __device__ bool stable_system(const double* eq, long long count, const DeviceIcpParams& p,
                               double* min_eig, double* cond) {
   if (count < p.min_corr) return false;
   double ev[6];
   sym_eig6(eq, ev);
+  return stable_from_eigenvalues(ev, p, min_eig, cond);
+}
+
+__device__ bool stable_from_eigenvalues(const double* ev, const DeviceIcpParams& p, double* min_eig, double* cond) {
   double mn = ev[0], mx = ev[0];
   for (int i = 0; i < 6; ++i) {
     if (!isfinite(ev[i])) return false;
@@ -585,6 +591,81 @@ __device__ void sym_eig6(const double* eq, double* ev) {
   for (int i = 0; i < 6; ++i) ev[i] = a[i][i];
 }
 
+// The same eigenvalues computed by one wave (the stability check at the end
+// of a level, where 64 threads are at hand).  Each sweep is 5 rounds of 3
+// rotations on disjoint index pairs (all 15 pairs once, round-robin order):
+// three lanes form the rotations of a round side by side, then 36 lanes
+// apply them to one element each, columns first, then rows.  Rounding
+// differs from the cyclic single-thread order; the eigenvalues agree to
+// ~1e-15 of the norm.
+namespace jacobi {
+constexpr int kP[5][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 1}, {0, 1, 4}, {0, 2, 3}};
+constexpr int kQ[5][3] = {{5, 4, 3}, {4, 5, 2}, {3, 4, 5}, {2, 3, 5}, {1, 5, 4}};
+}  // namespace jacobi
+
+__device__ void sym_eig6_wave(const double* eq, double* ev) {
+  __shared__ double A[6][6], B[6][6];
+  __shared__ double rc[3], rs[3];
+  __shared__ int converged;
+  const int lane = threadIdx.x;
+  const int r = lane / 6, c = lane % 6;
+  if (lane < 36) {
+    const int lo = min(r, c), hi = max(r, c);
+    A[r][c] = eq[lo * 6 - (lo * (lo - 1)) / 2 + (hi - lo)];  // packed upper triangle, row by row
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    if (lane == 0) {
+      double off = 0.0, diag = 0.0;
+      for (int d = 0; d < 6; ++d) diag += A[d][d] * A[d][d];
+      for (int pp = 0; pp < 6; ++pp)
+        for (int q = pp + 1; q < 6; ++q) off += A[pp][q] * A[pp][q];
+      converged = off <= 1.0e-32 * diag ? 1 : 0;
+    }
+    __syncthreads();
+    if (converged != 0) break;
+    for (int round = 0; round < 5; ++round) {
+      if (lane < 3) {
+        const int pp = jacobi::kP[round][lane], q = jacobi::kQ[round][lane];
+        double cc = 1.0, ss = 0.0;
+        if (A[pp][q] != 0.0) {
+          const double theta = (A[q][q] - A[pp][pp]) / (2.0 * A[pp][q]);
+          const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+          cc = 1.0 / sqrt(t * t + 1.0);
+          ss = t * cc;
+        }
+        rc[lane] = cc;
+        rs[lane] = ss;
+      }
+      __syncthreads();
+      // the rotation holding index j, and whether j is its first index
+      auto find = [&](int j, int& k, int& other, bool& first) {
+        k = 0;
+        for (int i = 0; i < 3; ++i) {
+          if (jacobi::kP[round][i] == j) { k = i; other = jacobi::kQ[round][i]; first = true; }
+          if (jacobi::kQ[round][i] == j) { k = i; other = jacobi::kP[round][i]; first = false; }
+        }
+      };
+      if (lane < 36) {  // columns: B = A J
+        int k, o;
+        bool first;
+        find(c, k, o, first);
+        B[r][c] = first ? rc[k] * A[r][c] - rs[k] * A[r][o] : rs[k] * A[r][o] + rc[k] * A[r][c];
+      }
+      __syncthreads();
+      if (lane < 36) {  // rows: A = J^T B
+        int k, o;
+        bool first;
+        find(r, k, o, first);
+        A[r][c] = first ? rc[k] * B[r][c] - rs[k] * B[o][c] : rs[k] * B[o][c] + rc[k] * B[r][c];
+      }
+      __syncthreads();
+    }
+  }
+  if (lane < 6) ev[lane] = A[lane][lane];
+  __syncthreads();
+}
+
 // Prepare a level: status cleared, model pose = running pose.
 __global__ void __launch_bounds__(64) icp_begin_level_kernel(IcpState* st, int level) {
   if (threadIdx.x != 0) return;
@@ -610,14 +691,18 @@ __global__ void __launch_bounds__(64) icp_begin_level_kernel(IcpState* st, int l
 
 // Stability check and result record (icp_optimizer.cpp:108-169).
 __global__ void __launch_bounds__(64) icp_finish_level_kernel(IcpState* st, DeviceIcpParams p, int level) {
-  if (threadIdx.x != 0) return;
+  __shared__ double ev[6];
   if (st->frame_failed) return;
+  const int status0 = st->status;
+  const bool check = (status0 == 0 || status0 == 1) && st->eq_count >= p.min_corr;
+  if (check) sym_eig6_wave(st->eq, ev);  // the whole wave
+  if (threadIdx.x != 0) return;
   kf_icp_result& r = st->results[level];
-  int status = st->status;
+  int status = status0;
   r.min_eigenvalue = 0.0;
   r.condition_number = 0.0;
   if (status == 0 || status == 1) {
-    const bool stable = stable_system(st->eq, st->eq_count, p, &r.min_eigenvalue, &r.condition_number);
+    const bool stable = check && stable_from_eigenvalues(ev, p, &r.min_eigenvalue, &r.condition_number);
     if (!stable && p.damping_mode == 0) status = 5;
   }
   r.status = status;
