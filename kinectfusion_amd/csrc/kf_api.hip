@@ -71,6 +71,7 @@ __global__ void frame_begin_kernel(IcpState* st, int init_mode) {
   if (threadIdx.x != 0) return;
   st->frame_failed = 0;
   st->failed_level = -1;
+  st->used_learned = 0;
   if (init_mode == KF_INIT_CONSTANT_VELOCITY && st->frames_done >= 2) {
     // T_k = T_{k-1} (T_{k-2}^-1 T_{k-1}) in FP64
     double inv2[16], rel[16], out[16];
@@ -334,10 +335,7 @@ void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing, const 
   int slot = 2;
   for (int l = c.levels - 1; l >= 0; --l) {
     LevelBuffers& L = ctx->lv[l];
-    if (iterations[l] > 0) {
-      launch_icp_begin_level(ctx, l);
-      launch_raycast(ctx, l, nullptr, nullptr, true);
-    }
+    if (iterations[l] > 0) launch_raycast(ctx, l, nullptr, nullptr, true, l);  // + begin the level
     if (timing) record(ctx, slot++);
     if (iterations[l] > 0) {
       launch_icp_level(ctx, L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k, iterations[l], p,

@@ -668,25 +668,7 @@ __device__ void sym_eig6_wave(const double* eq, double* ev) {
 
 // Prepare a level: status cleared, model pose = running pose.
 __global__ void __launch_bounds__(64) icp_begin_level_kernel(IcpState* st, int level) {
-  if (threadIdx.x != 0) return;
-  kf_icp_result& r = st->results[level];
-  r = kf_icp_result{};
-  r.status = -1;
-  if (st->frame_failed) return;
-  st->status = 0;
-  st->trials = 0;
-  st->rejected = 0;
-  st->best_cost = 0.0;
-  st->lambda = 0.0;
-  st->eq_count = 0;
-  st->best_count = 0;
-  st->update_t = 0.0;
-  st->update_r = 0.0;
-  for (int i = 0; i < 16; ++i) {
-    st->model_pose[i] = st->pose[i];
-    st->level_pose0[i] = st->pose[i];
-    st->best_pose[i] = st->pose[i];
-  }
+  if (threadIdx.x == 0) icp_begin_level(st, level);
 }
 
 // Stability check and result record (icp_optimizer.cpp:108-169).

@@ -289,6 +289,36 @@ struct IcpState {
   int pad2;
 };
 
+// Prepare a tracking level (one thread): the level's result reads "not run",
+// the trial state is cleared and the model pose is the running pose.  In a
+// failed frame nothing runs; a frame whose learned run was kept (marked
+// failed so that the fallback pass exits at once) keeps its results.
+__device__ inline void icp_begin_level(IcpState* st, int level) {
+  if (st->frame_failed != 0) {
+    if (st->used_learned == 0) {
+      st->results[level] = kf_icp_result{};
+      st->results[level].status = -1;
+    }
+    return;
+  }
+  st->results[level] = kf_icp_result{};
+  st->results[level].status = -1;
+  st->status = 0;
+  st->trials = 0;
+  st->rejected = 0;
+  st->best_cost = 0.0;
+  st->lambda = 0.0;
+  st->eq_count = 0;
+  st->best_count = 0;
+  st->update_t = 0.0;
+  st->update_r = 0.0;
+  for (int i = 0; i < 16; ++i) {
+    st->model_pose[i] = st->pose[i];
+    st->level_pose0[i] = st->pose[i];
+    st->best_pose[i] = st->pose[i];
+  }
+}
+
 struct DeviceIcpParams {
   long long min_corr;
   float max_dist, min_dot;
@@ -400,8 +430,10 @@ void launch_bilateral(kf_context* ctx, const uint16_t* raw);
 void launch_level_maps(kf_context* ctx, int level);
 void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state,
                       bool accumulate = false);
+// begin_level >= 0: the launch also prepares that tracking level
+// (icp_begin_level, by one thread) -- one launch fewer per level.
 void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
-                    bool skip_if_failed);
+                    bool skip_if_failed, int begin_level = -1);
 // single_launch: one kernel per trial (sweep, then the reduction and the
 // solve in the last workgroup behind device-scope fences) -- for passes that
 // rarely run and whose launches are mostly early exits (the learned warm

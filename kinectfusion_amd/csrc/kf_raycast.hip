@@ -31,8 +31,13 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
                                                        IcpState* state, float4* __restrict__ pts,
                                                        float4* __restrict__ nrm, uint32_t* __restrict__ col,
                                                        const uint32_t* __restrict__ skip_bits,
-                                                       const uint32_t* __restrict__ free_bits, BlockMap bm) {
+                                                       const uint32_t* __restrict__ free_bits, BlockMap bm,
+                                                       int begin_level) {
   Rigid pose = pose_arg;
+  // the level's preparation touches neither frame_failed nor pose, which
+  // every workgroup reads below
+  if (begin_level >= 0 && state != nullptr && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+    icp_begin_level(state, begin_level);
   if (state != nullptr) {
     if (state->frame_failed) return;
     pose = rigid_from_pose64(state->pose);
@@ -212,7 +217,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
 
 template <int Kind>
 void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState* state,
-              LevelBuffers& L) {
+              LevelBuffers& L, int begin_level) {
   using Raw = typename VoxelIO<Kind>::Raw;
   const dim3 grid((rp.w + 15) / 16, (rp.h + 15) / 16);
   if (hashed(ctx)) {
@@ -220,19 +225,21 @@ void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState*
         HashedStore<Kind>{hash_view(ctx), static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
         1.0F / ctx->grid.voxel};
     hipLaunchKernelGGL((raycast_kernel<Kind, HashedStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose,
-                       state, L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, block_map(ctx->grid));
+                       state, L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, block_map(ctx->grid),
+                       begin_level);
     return;
   }
   Sampler<Kind> vol{DenseStore<Kind>{static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
                     1.0F / ctx->grid.voxel};
   hipLaunchKernelGGL((raycast_kernel<Kind, DenseStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,
-                     L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, block_map(ctx->grid));
+                     L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, block_map(ctx->grid),
+                     begin_level);
 }
 
 }  // namespace
 
 void launch_raycast(kf_context* ctx, int level, const double* /*unused*/, const Rigid* pose_host,
-                    bool from_state) {
+                    bool from_state, int begin_level) {
   const kf_config& c = ctx->cfg;
   LevelBuffers& L = ctx->lv[level];
   RayParams rp{};
@@ -248,13 +255,13 @@ void launch_raycast(kf_context* ctx, int level, const double* /*unused*/, const 
   IcpState* state = from_state ? ctx->icp : nullptr;
   switch (c.voxel_store) {
     case KF_VOXEL_QUANTIZED:
-      launch_t<KF_VOXEL_QUANTIZED>(ctx, rp, pose, state, L);
+      launch_t<KF_VOXEL_QUANTIZED>(ctx, rp, pose, state, L, begin_level);
       break;
     case KF_VOXEL_BF16:
-      launch_t<KF_VOXEL_BF16>(ctx, rp, pose, state, L);
+      launch_t<KF_VOXEL_BF16>(ctx, rp, pose, state, L, begin_level);
       break;
     default:
-      launch_t<KF_VOXEL_FLOAT>(ctx, rp, pose, state, L);
+      launch_t<KF_VOXEL_FLOAT>(ctx, rp, pose, state, L, begin_level);
       break;
   }
   KF_HIP(hipGetLastError());

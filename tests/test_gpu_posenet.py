@@ -162,6 +162,10 @@ def test_learned_warm_start_tracks_and_gains(K, PN, torch_gpu):
             assert res.tracked == 1
             used += res.used_learned
             prop_err.append(res.proposal_translation_error)
+            if res.used_learned:
+                # a kept frame reports its level-0 run, not the skipped fallback
+                assert res.level[0].status in (0, 1) and res.level[0].trials >= 1
+                assert res.level[0].correspondences > 0
     traj = p.trajectory()
     motion = [np.linalg.norm((np.linalg.inv(traj[i - 1]) @ traj[i])[:3, 3]) for i in range(1, n)]
     gt_err = [np.linalg.norm(traj[i][:3, 3] - scene.pose(i * stride)[:3, 3]) for i in range(n)]
