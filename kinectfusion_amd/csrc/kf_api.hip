@@ -94,17 +94,21 @@ __global__ void frame_begin_kernel(IcpState* st, int init_mode) {
   }
 }
 
-// The trajectory slot is the device frame counter, so the same launch serves
-// every frame of a captured graph.
-__global__ void frame_end_kernel(IcpState* st, double* trajectory) {
-  if (threadIdx.x != 0) return;
-  const long long slot = st->frames_done;
-  for (int i = 0; i < 16; ++i) {
-    trajectory[slot * 16 + i] = st->pose[i];
-    st->prev2_pose[i] = st->prev_pose[i];
-    st->prev_pose[i] = st->pose[i];
+
+// The staging copies of a frame's inputs for the captured graph, both in one
+// launch (four pixels per thread).
+__global__ void __launch_bounds__(256) stage_inputs_kernel(const uint16_t* __restrict__ depth,
+                                                            uint16_t* __restrict__ depth_out,
+                                                            const uint32_t* __restrict__ color,
+                                                            uint32_t* __restrict__ color_out, size_t npx) {
+  const size_t i0 = (static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x) * 4;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const size_t i = i0 + k;
+    if (i >= npx) break;
+    depth_out[i] = depth[i];
+    if (color != nullptr) color_out[i] = color[i];
   }
-  st->frames_done += 1;
 }
 
 __global__ void set_identity_kernel(IcpState* st) {
@@ -398,8 +402,7 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     track_levels(ctx, to_device_params(opt), timing, ctx->cfg.icp_iterations);
   }
   ctx->timed_learned = timing && !first && learned_active(ctx);
-  launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true);
-  hipLaunchKernelGGL(frame_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->trajectory);
+  launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true, true);  // + the frame's end
   KF_HIP(hipGetLastError());
   if (ctx->cfg.init_mode == KF_INIT_LEARNED && ctx->prev_coarse != nullptr) {
     const LevelBuffers& L = ctx->lv[ctx->cfg.levels - 1];
@@ -826,9 +829,9 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
     } else {
       // tracking frames replay one captured graph over staging copies of the inputs
       const size_t npx = static_cast<size_t>(ctx->cfg.width) * ctx->cfg.height;
-      KF_HIP(hipMemcpyAsync(ctx->raw_depth, depth, npx * sizeof(uint16_t), hipMemcpyDeviceToDevice, ctx->stream));
-      if (color != nullptr)
-        KF_HIP(hipMemcpyAsync(ctx->raw_color, color, npx * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+      hipLaunchKernelGGL(stage_inputs_kernel, dim3(static_cast<unsigned>((npx + 1023) / 1024)), dim3(256), 0,
+                         ctx->stream, depth, ctx->raw_depth, color, ctx->raw_color, npx);
+      KF_HIP(hipGetLastError());
       const bool has_color = color != nullptr;
       if (ctx->graph_exec == nullptr || has_color != ctx->graph_color ||
           std::memcmp(&ctx->graph_opt, opt, sizeof(kf_icp_options)) != 0) {

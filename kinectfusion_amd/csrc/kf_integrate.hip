@@ -59,8 +59,15 @@ __global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restr
                                                           const float4* __restrict__ normals,
                                                           float4* __restrict__ pix,
                                                           float* __restrict__ tile_max,
-                                                          FrameParams f, float base_trunc) {
+                                                          FrameParams f, float base_trunc,
+                                                          uint32_t* __restrict__ counters,
+                                                          unsigned long long* __restrict__ fused) {
   __shared__ float red[256];
+  // the brick and fused counters start the pass at zero (read by later launches only)
+  if (blockIdx.x == 0 && blockIdx.y == 0) {
+    if (threadIdx.y == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0U;
+    if (threadIdx.y == 0 && threadIdx.x < 2) fused[threadIdx.x] = 0ULL;
+  }
   const int col = blockIdx.x * kTile + threadIdx.x, row = blockIdx.y * kTile + threadIdx.y;
   const int tid = threadIdx.y * kTile + threadIdx.x;
   float dmax = 0.0F;
@@ -533,10 +540,6 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
 }
 
 
-__global__ void zero_counters_kernel(uint32_t* c, unsigned long long* f) {
-  if (threadIdx.x < 4) c[threadIdx.x] = 0U;
-  if (threadIdx.x < 2) f[threadIdx.x] = 0ULL;
-}
 
 // running totals f[2..3] += per-call counts f[0..1]
 __global__ void accumulate_counts_kernel(unsigned long long* f) {
@@ -553,7 +556,20 @@ __device__ inline bool block_free(const uint32_t* __restrict__ parts, int fb) {
 // grid) hold only fused voxels reading tsdf exactly 1.
 __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restrict__ obs, uint32_t* __restrict__ skip,
                                                         const uint32_t* __restrict__ parts,
-                                                        uint32_t* __restrict__ free_bits, BlockMap bm, int nblocks) {
+                                                        uint32_t* __restrict__ free_bits, BlockMap bm, int nblocks,
+                                                        IcpState* end_state, double* trajectory) {
+  // end_state: this launch also closes the frame (one thread): the
+  // trajectory slot is the device frame counter, so the same launch serves
+  // every frame of a captured graph
+  if (end_state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
+    const long long slot = end_state->frames_done;
+    for (int i = 0; i < 16; ++i) {
+      trajectory[slot * 16 + i] = end_state->pose[i];
+      end_state->prev2_pose[i] = end_state->prev_pose[i];
+      end_state->prev_pose[i] = end_state->pose[i];
+    }
+    end_state->frames_done += 1;
+  }
   const int b = blockIdx.x * 256 + threadIdx.x;
   bool empty = false, free_nb = false;
   if (b < nblocks) {
@@ -949,11 +965,12 @@ void launch_fill_voxels(kf_context* ctx) {
   KF_HIP(hipGetLastError());
 }
 
-void launch_skip_map(kf_context* ctx) {
+void launch_skip_map(kf_context* ctx, bool end_frame) {
   const BlockMap bm = block_map(ctx->grid);
   const int nblocks = bm.bx * bm.by * bm.bz;
   hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
-                     ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks);
+                     ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks, end_frame ? ctx->icp : nullptr,
+                     ctx->trajectory);
   KF_HIP(hipGetLastError());
 }
 
@@ -979,7 +996,7 @@ void launch_rebuild_observed(kf_context* ctx) {
   launch_skip_map(ctx);
 }
 
-void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state,
+void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state, bool end_frame,
                       bool accumulate) {
   const kf_config& c = ctx->cfg;
   FrameParams f{};
@@ -1001,10 +1018,9 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
   // per-pixel and per-tile scratch is sized for the configured camera
   if (f.w > c.width || f.h > c.height)
     fail(KF_ERR_INVALID_ARGUMENT, "integrate: frame larger than the configured camera");
-  hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->counters, ctx->fused);
   hipLaunchKernelGGL(pixel_prep_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream,
                      fr.depth, reinterpret_cast<const float4*>(fr.normals), ctx->pix, ctx->tile_max,
-                     f, ctx->grid.trunc);
+                     f, ctx->grid.trunc, ctx->counters, ctx->fused);
   const bool color = ctx->colors != nullptr;
   const uint32_t* cimg = fr.color;
   const bool timing = c.stage_timing != 0 && from_state;
@@ -1035,7 +1051,7 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
         break;
     }
     if (timing) KF_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
-    launch_skip_map(ctx);
+    launch_skip_map(ctx, end_frame);
     if (count && accumulate)
       hipLaunchKernelGGL(accumulate_counts_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
     KF_HIP(hipGetLastError());
@@ -1063,7 +1079,7 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
       break;
   }
   if (timing) KF_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
-  launch_skip_map(ctx);
+  launch_skip_map(ctx, end_frame);
   if (count && accumulate)
     hipLaunchKernelGGL(accumulate_counts_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
   KF_HIP(hipGetLastError());

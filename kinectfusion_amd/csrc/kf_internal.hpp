@@ -429,7 +429,7 @@ void hip_check(hipError_t err, const char* what);
 void launch_bilateral(kf_context* ctx, const uint16_t* raw);
 void launch_level_maps(kf_context* ctx, int level);
 void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state,
-                      bool accumulate = false);
+                      bool end_frame = false, bool accumulate = false);
 // begin_level >= 0: the launch also prepares that tracking level
 // (icp_begin_level, by one thread) -- one launch fewer per level.
 void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
@@ -447,7 +447,9 @@ void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* l
                             float min_dot);
 void launch_icp_begin_level(kf_context* ctx, int level_index);
 void launch_fill_voxels(kf_context* ctx);
-void launch_skip_map(kf_context* ctx);      // skip bits from observed bits
+// skip and free bits from the observed and part bits; end_frame: the launch
+// also records the frame's pose (trajectory slot, previous poses, counter)
+void launch_skip_map(kf_context* ctx, bool end_frame = false);
 void launch_rebuild_observed(kf_context* ctx);  // observed bits from the voxels
 
 bool hashed(const kf_context* ctx);
