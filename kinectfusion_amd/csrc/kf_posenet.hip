@@ -264,14 +264,14 @@ __global__ void gemm_finish_kernel(Gemm g, int splits) {
 // ---------------------------------------------------------------------------
 // Small GEMM (few output tiles, K up to a few thousand: the layers of a pose
 // proposal, and the forward layers of small batches).  One workgroup per
-// 16 x 16 output tile; its kSmallWaves waves split the K steps (16 wide)
+// 16 x 16 output tile; its Waves waves (8, or 16 for long reductions) split the K steps (16 wide)
 // between them and load their operand fragments straight from global memory
 // into registers, kSmallChunk steps at a time, so that a layer costs one or
 // two memory round trips instead of one per K step.  Lane l holds
 // A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] (j = 0..3) of a
 // step and D[4 (l / 16) + r][l % 16] of the tile.  The per-wave tiles are
 // added in wave order through LDS, so the result does not depend on timing.
-constexpr int kSmallWaves = 8, kSmallChunk = 8;
+constexpr int kSmallChunk = 10;
 
 // A(m, k) of the im2col matrix for a row whose receptive-field origin is
 // (oy, ox) of pair ob (oy < -2 marks a row past M).
@@ -289,8 +289,9 @@ __device__ inline uint16_t conv_a(const ConvView& v, int oy, int ox, int ob, int
 
 __device__ inline short as_short(uint16_t h) { return static_cast<short>(h); }
 
-__global__ void __launch_bounds__(kSmallWaves * 64) gemm_small_kernel(Gemm g, int vec_a, int vec_b) {
-  __shared__ float red[kSmallWaves][16][17];
+template <int Waves>
+__global__ void __launch_bounds__(Waves * 64) gemm_small_kernel(Gemm g, int vec_a, int vec_b) {
+  __shared__ float red[Waves][16][17];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int li = lane & 15, lq = lane >> 4;
   const int m0 = blockIdx.x * 16, n0 = blockIdx.y * 16;
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(kSmallWaves * 64) gemm_small_kernel(Gemm g, in
     ob = t / g.conv.Ho;
   }
   const int steps = (g.K + 15) / 16;
-  const int per = (steps + kSmallWaves - 1) / kSmallWaves;
+  const int per = (steps + Waves - 1) / Waves;
   const int s_begin = wave * per, s_end = min(steps, s_begin + per);
   f32x4 acc = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
   for (int c0 = s_begin; c0 < s_end; c0 += kSmallChunk) {
@@ -364,7 +365,7 @@ __global__ void __launch_bounds__(kSmallWaves * 64) gemm_small_kernel(Gemm g, in
     if (om < g.M && on < g.N) {
       float s = 0.0F;
 #pragma unroll
-      for (int w = 0; w < kSmallWaves; ++w) s += red[w][row][col];
+      for (int w = 0; w < Waves; ++w) s += red[w][row][col];
       gemm_store(g, om, on, s);
     }
   }
@@ -572,8 +573,13 @@ void gemm(kf_posenet* net, hipStream_t s, Gemm g) {
     }
     const int vec_b = g.sbk == 1 && g.sbn % 4 == 0 && aligned8(g.b) ? 1 : 0;
     g.partial = nullptr;
-    hipLaunchKernelGGL(gemm_small_kernel, dim3((g.M + 15) / 16, (g.N + 15) / 16), dim3(kSmallWaves * 64), 0, s, g,
-                       vec_a, vec_b);
+    // a reduction longer than 8 waves x kSmallChunk steps (fc1: K = 2560) takes 16 waves: one read round
+    const dim3 grid((g.M + 15) / 16, (g.N + 15) / 16);
+    if ((g.K + 15) / 16 > 8 * kSmallChunk) {
+      hipLaunchKernelGGL(gemm_small_kernel<16>, grid, dim3(16 * 64), 0, s, g, vec_a, vec_b);
+    } else {
+      hipLaunchKernelGGL(gemm_small_kernel<8>, grid, dim3(8 * 64), 0, s, g, vec_a, vec_b);
+    }
     KF_HIP(hipGetLastError());
     return;
   }
