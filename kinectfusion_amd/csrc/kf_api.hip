@@ -561,10 +561,9 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->raw_depth = dalloc<uint16_t>(npx);
     ctx->raw_color = dalloc<uint32_t>(npx);
     {
-      // 16 px tiles, then 4 x 4 and 16 x 16 groups of them, then the frame maximum
+      // the depth maximum of each 16 px tile
       const int tx = (c.width + kTile - 1) / kTile, ty = (c.height + kTile - 1) / kTile;
-      const int t1x = (tx + 3) / 4, t1y = (ty + 3) / 4, t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
-      ctx->tile_max = dalloc<float>(static_cast<size_t>(tx) * ty + t1x * t1y + t2x * t2y + 1);
+      ctx->tile_max = dalloc<float>(static_cast<size_t>(tx) * ty);
     }
     if (c.storage == KF_STORAGE_DENSE) {
       const size_t nb = static_cast<size_t>((c.resolution[0] + kBrickX - 1) / kBrickX) *

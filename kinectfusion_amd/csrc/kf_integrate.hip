@@ -7,16 +7,18 @@
 //     measured depth, |K^-1 [u v 1]| (lambda), the rule's observation weight
 //     (angle rule: cos(theta)/depth from the level-0 normal), the truncation
 //     band; plus the maximum usable depth of every 16 x 16 pixel tile.
-//  2. brick_cull  -- one thread per 64 x 4 x 4 voxel brick: a brick is kept
+//  2. brick_cull  -- one thread per 16 x 4 x 8 voxel brick: a brick is kept
 //     unless no voxel of it can project into the image in front of the
 //     camera, or its nearest voxel lies farther than the deepest depth of
 //     the tiles it covers plus the truncation band.  The bounds carry
 //     margins so that no voxel the full sweep would fuse is ever culled: the
 //     result is bit-identical to the full sweep.  Kept bricks are compacted
-//     into a list with one atomic per wave.
-//  3. integrate   -- persistent workgroups walk the list; a wave covers one
-//     64-voxel x-row of a brick, so voxel loads and stores are contiguous
-//     (512 B per wave for 8 B voxels, 256 B for 4 B voxels).
+//     into a list in brick order, one atomic per workgroup.
+//  3. integrate   -- persistent waves walk the list, one brick per wave: a
+//     lane owns an (x, y) column of 8 voxels, and each z-slice of the brick
+//     is 4 rows of 16 contiguous voxels.
+#include <type_traits>
+
 #include "kf_internal.hpp"
 
 namespace kf {
@@ -62,7 +64,7 @@ __global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restr
                                                           FrameParams f, float base_trunc,
                                                           uint32_t* __restrict__ counters,
                                                           unsigned long long* __restrict__ fused) {
-  __shared__ float red[256];
+  __shared__ float red[4];
   // the brick and fused counters start the pass at zero (read by later launches only)
   if (blockIdx.x == 0 && blockIdx.y == 0) {
     if (threadIdx.y == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0U;
@@ -96,152 +98,209 @@ __global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restr
     }
     pix[i] = rec;
   }
-  red[tid] = dmax;
+  // tile maximum: a butterfly in each wave, then the 4 wave maxima
+  float m = dmax;
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((tid & 63) == 0) red[tid >> 6] = m;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) red[tid] = fmaxf(red[tid], red[tid + s]);
-    __syncthreads();
-  }
-  if (tid == 0) tile_max[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+  if (tid == 0) tile_max[blockIdx.y * gridDim.x + blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
-// Coarser depth-maximum levels for the cull: 4 x 4 tiles of 16 px (64 px)
-// and 16 x 16 tiles (256 px), plus the frame maximum.  One workgroup.
-__global__ void __launch_bounds__(256) tile_pyramid_kernel(float* __restrict__ tile_max, int tiles_x, int tiles_y) {
-  float* l1 = tile_max + tiles_x * tiles_y;
-  const int t1x = (tiles_x + 3) / 4, t1y = (tiles_y + 3) / 4;
-  float* l2 = l1 + t1x * t1y;
-  const int t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
-  float* all = l2 + t2x * t2y;
-  for (int i = threadIdx.x; i < t1x * t1y; i += 256) {
-    const int x = i % t1x, y = i / t1x;
-    float m = 0.0F;
-    for (int yy = y * 4; yy < min(y * 4 + 4, tiles_y); ++yy)
-      for (int xx = x * 4; xx < min(x * 4 + 4, tiles_x); ++xx) m = fmaxf(m, tile_max[yy * tiles_x + xx]);
-    l1[i] = m;
+// Depth maxima the cull reads: the 16 px tiles (written by the pixel pass)
+// and, built by each cull workgroup in LDS, 4 x 4 groups of tiles (64 px),
+// 4 x 4 groups of those (256 px) and the frame maximum.  The group levels
+// hold bit patterns (for floats >= 0 the unsigned order is the float order)
+// so that LDS atomic maxima build them; l1 is null when they do not fit.
+struct TilePyramid {
+  const float* l0;
+  const uint32_t* l1;
+  const uint32_t* l2;
+  int tx, ty, t1x, t2x;
+  float frame_max;
+};
+
+constexpr int kPyramidLdsBytes = 32768;  // LDS for the group levels
+
+// Maximum of the entries [x0, x1] x [y0, y1] (at most 4 x 4) of a level w wide.
+template <typename T>
+__device__ inline float gather4x4(const T* lv, int w, int x0, int x1, int y0, int y1) {
+  float m = 0.0F;
+#pragma unroll
+  for (int dy = 0; dy < 4; ++dy) {
+    const int y = min(y0 + dy, y1);
+#pragma unroll
+    for (int dx = 0; dx < 4; ++dx) {
+      const T e = lv[y * w + min(x0 + dx, x1)];
+      if constexpr (sizeof(T) == 4 && !std::is_same_v<T, float>) m = fmaxf(m, __uint_as_float(e));
+      else m = fmaxf(m, e);
+    }
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < t2x * t2y; i += 256) {
-    const int x = i % t2x, y = i / t2x;
-    float m = 0.0F;
-    for (int yy = y * 4; yy < min(y * 4 + 4, t1y); ++yy)
-      for (int xx = x * 4; xx < min(x * 4 + 4, t1x); ++xx) m = fmaxf(m, l1[yy * t1x + xx]);
-    l2[i] = m;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float m = 0.0F;
-    for (int i = 0; i < t2x * t2y; ++i) m = fmaxf(m, l2[i]);
-    *all = m;
-  }
+  return m;
 }
 
 // Maximum usable depth over the 16 px tiles [tx0, tx1] x [ty0, ty1], read at
-// the coarsest level that still needs at most 4 x 4 lookups (a coarser tile
-// covers more pixels, so the bound stays conservative).
-__device__ inline float region_max(const float* __restrict__ tile_max, int tiles_x, int tiles_y, int tx0, int tx1,
-                                   int ty0, int ty1) {
-  const int t1x = (tiles_x + 3) / 4, t1y = (tiles_y + 3) / 4;
-  const int t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
-  const float* l1 = tile_max + tiles_x * tiles_y;
-  const float* l2 = l1 + t1x * t1y;
-  const float* all = l2 + t2x * t2y;
-  const float* lv = tile_max;
-  int w = tiles_x;
-  for (int level = 0; level < 3; ++level) {
-    if ((tx1 - tx0 + 1) * (ty1 - ty0 + 1) <= 16) {
-      float m = 0.0F;
-      for (int y = ty0; y <= ty1; ++y)
-        for (int x = tx0; x <= tx1; ++x) m = fmaxf(m, lv[y * w + x]);
-      return m;
-    }
-    tx0 >>= 2;
-    tx1 >>= 2;
-    ty0 >>= 2;
-    ty1 >>= 2;
-    lv = level == 0 ? l1 : l2;
-    w = level == 0 ? t1x : t2x;
-  }
-  (void)t2y;
-  return *all;
+// the finest level where the range spans at most 4 x 4 entries (a coarser
+// entry covers more pixels, so the bound stays conservative).  The 16 reads
+// are issued together, clamped into the range (a repeated entry does not
+// change the maximum).
+__device__ inline float region_max(const TilePyramid& p, int tx0, int tx1, int ty0, int ty1) {
+  if (tx1 - tx0 < 4 && ty1 - ty0 < 4) return gather4x4(p.l0, p.tx, tx0, tx1, ty0, ty1);
+  if (p.l1 == nullptr) return p.frame_max;
+  tx0 >>= 2;
+  tx1 >>= 2;
+  ty0 >>= 2;
+  ty1 >>= 2;
+  if (tx1 - tx0 < 4 && ty1 - ty0 < 4) return gather4x4(p.l1, p.t1x, tx0, tx1, ty0, ty1);
+  tx0 >>= 2;
+  tx1 >>= 2;
+  ty0 >>= 2;
+  ty1 >>= 2;
+  if (tx1 - tx0 < 4 && ty1 - ty0 < 4) return gather4x4(p.l2, p.t2x, tx0, tx1, ty0, ty1);
+  return p.frame_max;
 }
 
-// 2. conservative brick cull.
+// 2. conservative brick cull: whether any voxel of brick (bx, by, bz) can be
+// fused by this frame.  The brick's 8 corner centres bound it; in front of
+// the camera, its image lies in the hull of their projections (a convex
+// box projects to the hull of its corners), widened by a margin; every
+// voxel behind the deepest depth of that image region by more than the
+// truncation is rejected by the fusion test.
+__device__ inline bool brick_may_fuse(const Grid& g, const FrameParams& f, const Rigid& w2c, const TilePyramid& tp,
+                                      int bx, int by, int bz) {
+  const int tiles_x = tp.tx, tiles_y = tp.ty;
+  bool keep;
+  const int xs[2] = {bx * kBrickX, min(bx * kBrickX + kBrickX - 1, g.nx - 1)};
+  const int ys[2] = {by * kBrickY, min(by * kBrickY + kBrickY - 1, g.ny - 1)};
+  const int zs[2] = {bz * kBrickZ, min(bz * kBrickZ + kBrickZ - 1, g.nz - 1)};
+  float zmin = 3.0e38F, zmax = -3.0e38F, umin = 3.0e38F, umax = -3.0e38F, vmin = 3.0e38F,
+        vmax = -3.0e38F;
+  V3 cam[8];
+  for (int c = 0; c < 8; ++c) {
+    cam[c] = xform(w2c, g.center(xs[c & 1], ys[(c >> 1) & 1], zs[c >> 2]));
+    zmin = fminf(zmin, cam[c].z);
+    zmax = fmaxf(zmax, cam[c].z);
+  }
+  keep = zmax > -1.0e-4F;
+  int tx0 = 0, tx1 = tiles_x - 1, ty0 = 0, ty1 = tiles_y - 1;
+  if (keep && zmin > 1.0e-3F) {
+    for (int c = 0; c < 8; ++c) {
+      // hardware reciprocal: its error (2^-21 relative) is far inside the margin below
+      const float rz = __builtin_amdgcn_rcpf(cam[c].z);
+      const float u = ((f.k.fx * cam[c].x) * rz) + f.k.cx, v = ((f.k.fy * cam[c].y) * rz) + f.k.cy;
+      umin = fminf(umin, u);
+      umax = fmaxf(umax, u);
+      vmin = fminf(vmin, v);
+      vmax = fmaxf(vmax, v);
+    }
+    // a voxel lands on pixel round(u): usable columns are [0, w); keep a
+    // two-pixel margin against rounding of the hull vertices.
+    umin -= 2.0F;
+    vmin -= 2.0F;
+    umax += 2.0F;
+    vmax += 2.0F;
+    if (umax < -0.5F || vmax < -0.5F || umin > static_cast<float>(f.w) - 0.5F ||
+        vmin > static_cast<float>(f.h) - 0.5F) {
+      keep = false;
+    } else {
+      tx0 = max(0, static_cast<int>(floorf((fmaxf(umin, -1.0F) + 0.5F) / kTile)));
+      ty0 = max(0, static_cast<int>(floorf((fmaxf(vmin, -1.0F) + 0.5F) / kTile)));
+      tx1 = min(tiles_x - 1, static_cast<int>(floorf((fminf(umax, 1.0e6F) + 0.5F) / kTile)));
+      ty1 = min(tiles_y - 1, static_cast<int>(floorf((fminf(vmax, 1.0e6F) + 0.5F) / kTile)));
+    }
+  }
+  if (keep && (tx0 > tx1 || ty0 > ty1)) keep = false;
+  if (keep) {
+    const float dmax = region_max(tp, tx0, tx1, ty0, ty1);
+    if (!(dmax > 0.0F)) {
+      keep = false;
+    } else if (zmin > 1.0e-3F) {
+      // |p| / lambda >= z (1 - 0.71 / fx) for a voxel rounded to its pixel;
+      // fusion needs |p| / lambda <= d + truncation.
+      const float trunc = f.dist_scaled ? g.trunc + f.trunc_scale * dmax : g.trunc;
+      const float shrink = 1.0F - 1.0F / fmaxf(f.k.fx, 1.0F);
+      if (zmin * shrink > dmax + trunc + 1.0e-3F) keep = false;
+    }
+  }
+  return keep;
+}
+
+// One thread per brick for kCullBricks bricks in a row; the workgroup's kept
+// bricks take one range of the list (one atomic per workgroup), in brick
+// order.
+constexpr int kCullBricks = 4;
+
 __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
                                                           const float* __restrict__ tile_max,
                                                           int tiles_x, int tiles_y,
                                                           uint32_t* __restrict__ list,
-                                                          uint32_t* __restrict__ count) {
+                                                          uint32_t* __restrict__ count, int group_levels) {
+  extern __shared__ uint32_t levels[];  // l1 | l2 when group_levels
+  __shared__ uint32_t run_start[kCullBricks * 4];
+  __shared__ uint32_t wg_base;
+  __shared__ float wave_max[4];
   const int bnx = (g.nx + kBrickX - 1) / kBrickX, bny = (g.ny + kBrickY - 1) / kBrickY;
   const int bnz = (g.nz + kBrickZ - 1) / kBrickZ;
-  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = bnx * bny * bnz;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
-  bool keep = false;
-  if (b < static_cast<uint32_t>(bnx * bny * bnz)) {
-    const int bx = b % bnx, by = (b / bnx) % bny, bz = b / (bnx * bny);
-    const int xs[2] = {bx * kBrickX, min(bx * kBrickX + kBrickX - 1, g.nx - 1)};
-    const int ys[2] = {by * kBrickY, min(by * kBrickY + kBrickY - 1, g.ny - 1)};
-    const int zs[2] = {bz * kBrickZ, min(bz * kBrickZ + kBrickZ - 1, g.nz - 1)};
-    float zmin = 3.0e38F, zmax = -3.0e38F, umin = 3.0e38F, umax = -3.0e38F, vmin = 3.0e38F,
-          vmax = -3.0e38F;
-    V3 cam[8];
-    for (int c = 0; c < 8; ++c) {
-      cam[c] = xform(w2c, g.center(xs[c & 1], ys[(c >> 1) & 1], zs[c >> 2]));
-      zmin = fminf(zmin, cam[c].z);
-      zmax = fmaxf(zmax, cam[c].z);
-    }
-    keep = zmax > -1.0e-4F;
-    int tx0 = 0, tx1 = tiles_x - 1, ty0 = 0, ty1 = tiles_y - 1;
-    if (keep && zmin > 1.0e-3F) {
-      for (int c = 0; c < 8; ++c) {
-        float u, v;
-        project(f.k, cam[c], u, v);
-        umin = fminf(umin, u);
-        umax = fmaxf(umax, u);
-        vmin = fminf(vmin, v);
-        vmax = fmaxf(vmax, v);
-      }
-      // a voxel lands on pixel round(u): usable columns are [0, w); keep a
-      // two-pixel margin against rounding of the hull vertices.
-      umin -= 2.0F;
-      vmin -= 2.0F;
-      umax += 2.0F;
-      vmax += 2.0F;
-      if (umax < -0.5F || vmax < -0.5F || umin > static_cast<float>(f.w) - 0.5F ||
-          vmin > static_cast<float>(f.h) - 0.5F) {
-        keep = false;
-      } else {
-        tx0 = max(0, static_cast<int>(floorf((fmaxf(umin, -1.0F) + 0.5F) / kTile)));
-        ty0 = max(0, static_cast<int>(floorf((fmaxf(vmin, -1.0F) + 0.5F) / kTile)));
-        tx1 = min(tiles_x - 1, static_cast<int>(floorf((fminf(umax, 1.0e6F) + 0.5F) / kTile)));
-        ty1 = min(tiles_y - 1, static_cast<int>(floorf((fminf(vmax, 1.0e6F) + 0.5F) / kTile)));
-      }
-    }
-    if (keep && (tx0 > tx1 || ty0 > ty1)) keep = false;
-    if (keep) {
-      const float dmax = region_max(tile_max, tiles_x, tiles_y, tx0, tx1, ty0, ty1);
-      if (!(dmax > 0.0F)) {
-        keep = false;
-      } else if (zmin > 1.0e-3F) {
-        // |p| / lambda >= z (1 - 0.71 / fx) for a voxel rounded to its pixel;
-        // fusion needs |p| / lambda <= d + truncation.
-        const float trunc = f.dist_scaled ? g.trunc + f.trunc_scale * dmax : g.trunc;
-        const float shrink = 1.0F - 1.0F / fmaxf(f.k.fx, 1.0F);
-        if (zmin * shrink > dmax + trunc + 1.0e-3F) keep = false;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // the workgroup's depth pyramid
+  const int t1x = (tiles_x + 3) / 4, t1y = (tiles_y + 3) / 4, t2x = (t1x + 3) / 4, t2y = (t1y + 3) / 4;
+  const int n1 = t1x * t1y, n_tiles = tiles_x * tiles_y;
+  if (group_levels != 0)
+    for (int i = static_cast<int>(threadIdx.x); i < n1 + t2x * t2y; i += 256) levels[i] = 0U;
+  __syncthreads();
+  float fmax_t = 0.0F;
+  for (int i0 = 0; i0 < n_tiles; i0 += 8 * 256) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = tile_max[min(i0 + k * 256 + static_cast<int>(threadIdx.x), n_tiles - 1)];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int i = i0 + k * 256 + static_cast<int>(threadIdx.x);
+      fmax_t = fmaxf(fmax_t, v[k]);
+      if (group_levels != 0 && i < n_tiles && v[k] > 0.0F) {
+        const int x = i % tiles_x, y = i / tiles_x;
+        atomicMax(&levels[(y >> 2) * t1x + (x >> 2)], __float_as_uint(v[k]));
+        atomicMax(&levels[n1 + (y >> 4) * t2x + (x >> 4)], __float_as_uint(v[k]));
       }
     }
   }
-  // wave-aggregated append
-  const unsigned long long mask = __ballot(keep);
-  const int lane = threadIdx.x & 63;
-  uint32_t base = 0;
-  if (lane == 0 && mask != 0ULL) base = atomicAdd(count, static_cast<uint32_t>(__popcll(mask)));
-  base = __shfl(base, 0);
-  if (keep) {
-    const unsigned long long below = mask & ((1ULL << lane) - 1ULL);
-    list[base + __popcll(below)] = pack_brick(static_cast<int>(b % bnx), static_cast<int>((b / bnx) % bny),
-                                              static_cast<int>(b / (bnx * bny)));
+  for (int off = 32; off > 0; off >>= 1) fmax_t = fmaxf(fmax_t, __shfl_xor(fmax_t, off));
+  if (lane == 0) wave_max[wave] = fmax_t;
+  __syncthreads();
+  TilePyramid tp{tile_max, group_levels != 0 ? levels : nullptr, levels + n1, tiles_x, tiles_y, t1x, t2x,
+                 fmaxf(fmaxf(wave_max[0], wave_max[1]), fmaxf(wave_max[2], wave_max[3]))};
+  unsigned long long kept[kCullBricks];
+#pragma unroll
+  for (int k = 0; k < kCullBricks; ++k) {
+    const int b = (static_cast<int>(blockIdx.x) * kCullBricks + k) * 256 + static_cast<int>(threadIdx.x);
+    const bool keep = b < nb && brick_may_fuse(g, f, w2c, tp, b % bnx, (b / bnx) % bny, b / (bnx * bny));
+    kept[k] = __ballot(keep);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < kCullBricks; ++k) run_start[k * 4 + wave] = static_cast<uint32_t>(__popcll(kept[k]));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (int i = 0; i < kCullBricks * 4; ++i) {
+      const uint32_t c = run_start[i];
+      run_start[i] = total;
+      total += c;
+    }
+    wg_base = total != 0U ? atomicAdd(count, total) : 0U;
+  }
+  __syncthreads();
+  const unsigned long long below = (1ULL << lane) - 1ULL;
+#pragma unroll
+  for (int k = 0; k < kCullBricks; ++k) {
+    if (((kept[k] >> lane) & 1ULL) != 0ULL) {
+      const int b = (static_cast<int>(blockIdx.x) * kCullBricks + k) * 256 + static_cast<int>(threadIdx.x);
+      list[wg_base + run_start[k * 4 + wave] + static_cast<uint32_t>(__popcll(kept[k] & below))] =
+          pack_brick(b % bnx, (b / bnx) % bny, b / (bnx * bny));
+    }
   }
 }
 
@@ -470,25 +529,23 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
                                                          unsigned long long* __restrict__ fused,
                                                          uint32_t* __restrict__ obs_bits, BlockMap bm,
                                                          uint32_t* __restrict__ part_bits) {
-  // A brick is 8 blocks along x of one y-half; a thread's column of 8
-  // voxels is both z-halves of its block.  parts_*[h] bit j: block j's part
-  // (y-half, z-half h) had every voxel ok / complete in every row.
-  __shared__ uint32_t parts_ok[2], parts_complete[2];
+  // One wave per listed brick, the waves independent of each other: lane
+  // (x = lane % 16, y = lane / 16) owns the brick's z-column of 8 voxels,
+  // both z-halves of its block.  Lanes 8j .. 8j + 7 of each 16 are block j's
+  // row (j = 0, 1), so kBlockRows selects block j's 32 lanes of a ballot.
+  constexpr unsigned long long kBlockRows = 0x00FF00FF00FF00FFULL;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
   const uint32_t n = *count;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int tx = lane & 15, ty = lane >> 4;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
   unsigned int my_fused = 0, my_color = 0;
-  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+  for (uint32_t li = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); li < n; li += waves) {
     const uint32_t b = list[li];
-    const int bx = static_cast<int>(b & 0xFFU), by = static_cast<int>((b >> 8U) & 0xFFFU),
-              bz = static_cast<int>(b >> 20U);
+    const int bx = static_cast<int>(b & 0xFFU), by = static_cast<int>((b >> 8U) & 0x3FFU),
+              bz = static_cast<int>(b >> 18U);
     const int x = bx * kBrickX + tx, y = by * kBrickY + ty, z0 = bz * kBrickZ;
-    if (threadIdx.x < 2) {
-      parts_ok[threadIdx.x] = 0xFFU;
-      parts_complete[threadIdx.x] = 0xFFU;
-    }
-    __syncthreads();
     bool any0 = false, any1 = false, ok0 = false, ok1 = false, cpl0 = false, cpl1 = false;
     if (x < g.nx && y < g.ny) {
       any0 = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors, x, y, z0, my_fused,
@@ -496,36 +553,21 @@ __global__ void __launch_bounds__(256) integrate_kernel(Grid g, FrameParams f,
       any1 = integrate_column<Kind, Color, Count>(g, f, w2c, pix, color_img, vox, colors, x, y, z0 + kBrickZ / 2,
                                                   my_fused, my_color, ok1, cpl1);
     }
-    // Lanes 8j..8j+7 of a wave are block j's row: one ballot per flag; lane
-    // j < 8 folds its block's byte of each.
     const unsigned long long om0 = __ballot(ok0), cm0 = __ballot(cpl0), om1 = __ballot(ok1),
                              cm1 = __ballot(cpl1), am = __ballot(any0 || any1);
-    if (tx < 8) {
-      const int j = tx;
-      auto full = [&](unsigned long long m) { return ((m >> (8 * j)) & 0xFFULL) == 0xFFULL; };
-      const uint32_t bit = 1U << j;
-      if (!full(om0)) atomicAnd(&parts_ok[0], ~bit);
-      if (!full(cm0)) atomicAnd(&parts_complete[0], ~bit);
-      if (!full(om1)) atomicAnd(&parts_ok[1], ~bit);
-      if (!full(cm1)) atomicAnd(&parts_complete[1], ~bit);
-      const int blk_x = bx * 8 + j;
-      if (((am >> (8 * j)) & 0xFFULL) != 0ULL && blk_x < bm.bx) {
-        const int fb = bm.flat(blk_x, (by * kBrickY) >> 3, z0 >> 3);
-        atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
-      }
-    }
-    __syncthreads();
-    if (threadIdx.x < 16) {
-      // part (y-half, z-half h) of block j
-      const int j = threadIdx.x & 7, h = threadIdx.x >> 3;
-      const int blk_x = bx * 8 + j;
+    if (lane < 4) {
+      // lane = z-half h * 2 + block j: part (y-half, h) of block j, and j's observed bit
+      const int j = lane & 1, h = lane >> 1;
+      const int blk_x = bx * 2 + j;
       if (blk_x < bm.bx) {
+        const unsigned long long rows = kBlockRows << (8 * j);
+        const int fb = bm.flat(blk_x, (by * kBrickY) >> 3, z0 >> 3);
+        const bool ok = ((h != 0 ? om1 : om0) & rows) == rows, cpl = ((h != 0 ? cm1 : cm0) & rows) == rows;
         const int part = (((by * kBrickY) >> 2) & 1) * 2 + h;
-        const bool ok = ((parts_ok[h] >> j) & 1U) != 0U, cpl = ((parts_complete[h] >> j) & 1U) != 0U;
-        if (!ok || cpl) set_part(part_bits, bm.flat(blk_x, (by * kBrickY) >> 3, z0 >> 3), part, ok);
+        if (!ok || cpl) set_part(part_bits, fb, part, ok);
+        if (h == 0 && (am & rows) != 0ULL) atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
       }
     }
-    __syncthreads();
   }
   if (Count) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -552,6 +594,18 @@ __device__ inline bool block_free(const uint32_t* __restrict__ parts, int fb) {
   return ((parts[fb >> 3] >> ((fb & 7) * 4)) & 0xFU) == 0xFU;
 }
 
+// The end of a frame: its pose goes to the trajectory slot of the device
+// frame counter and becomes the previous pose.
+__device__ inline void close_frame(IcpState* st, double* trajectory) {
+  const long long slot = st->frames_done;
+  for (int i = 0; i < 16; ++i) {
+    trajectory[slot * 16 + i] = st->pose[i];
+    st->prev2_pose[i] = st->prev_pose[i];
+    st->prev_pose[i] = st->pose[i];
+  }
+  st->frames_done += 1;
+}
+
 // free bit of a block = the block and its 26 neighbours (all inside the
 // grid) hold only fused voxels reading tsdf exactly 1.
 __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restrict__ obs, uint32_t* __restrict__ skip,
@@ -561,15 +615,7 @@ __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restric
   // end_state: this launch also closes the frame (one thread): the
   // trajectory slot is the device frame counter, so the same launch serves
   // every frame of a captured graph
-  if (end_state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) {
-    const long long slot = end_state->frames_done;
-    for (int i = 0; i < 16; ++i) {
-      trajectory[slot * 16 + i] = end_state->pose[i];
-      end_state->prev2_pose[i] = end_state->prev_pose[i];
-      end_state->prev_pose[i] = end_state->pose[i];
-    }
-    end_state->frames_done += 1;
-  }
+  if (end_state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) close_frame(end_state, trajectory);
   const int b = blockIdx.x * 256 + threadIdx.x;
   bool empty = false, free_nb = false;
   if (b < nblocks) {
@@ -611,6 +657,75 @@ __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restric
   const int nwords = (nblocks + 31) >> 5;
   if (lane == 0 && word0 < nwords) skip[word0] = static_cast<uint32_t>(m);
   if (lane == 32 && word0 + 1 < nwords) skip[word0 + 1] = static_cast<uint32_t>(m >> 32);
+}
+
+// Bit i of the result: block i of the 32 blocks whose part nibbles are the
+// four words p has every part set.
+__device__ inline uint32_t full_blocks(uint4 p) {
+  uint32_t m = 0;
+  const uint32_t w[4] = {p.x, p.y, p.z, p.w};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t x = w[q] & (w[q] >> 1);
+    x = x & (x >> 2) & 0x11111111U;  // bit 4k: nibble k is 0xF
+    x = (x | (x >> 3)) & 0x03030303U;
+    x = (x | (x >> 6)) & 0x000F000FU;
+    x = (x | (x >> 12)) & 0x000000FFU;
+    m |= x << (8 * q);
+  }
+  return m;
+}
+
+// The same maps a word at a time, when every row of blocks (fixed y, z)
+// fills whole 32-bit words (bm.bx a multiple of 32): one thread per word.
+// skip = no observed block among the 3 x 3 x 3 around (the rows around are
+// OR-ed, then spread by one block along x); free = every block of the
+// 3 x 3 x 3 full (the rows around are AND-ed, then shrunk by one block along
+// x; blocks at the grid's faces are never free).
+__global__ void __launch_bounds__(64) skip_map_rows_kernel(const uint32_t* __restrict__ obs,
+                                                             uint32_t* __restrict__ skip,
+                                                             const uint32_t* __restrict__ parts,
+                                                             uint32_t* __restrict__ free_bits, BlockMap bm,
+                                                             IcpState* end_state, double* trajectory) {
+  if (end_state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) close_frame(end_state, trajectory);
+  const int wpr = bm.bx >> 5;  // words per row
+  const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (t >= wpr * bm.by * bm.bz) return;
+  const int wx = t % wpr, row = t / wpr, y = row % bm.by, z = row / bm.by;
+  const bool interior = y > 0 && z > 0 && y + 1 < bm.by && z + 1 < bm.bz;
+  // every read of the 9 rows around is issued at once: a row outside the
+  // grid, or a neighbouring word outside the row, reads the thread's own
+  // word and is masked out
+  uint32_t o[9], o_lo[9], o_hi[9], p_lo[9], p_hi[9];
+  uint4 p[9];
+  bool in[9];
+  const auto* parts4 = reinterpret_cast<const uint4*>(parts);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    const int yy = y + k % 3 - 1, zz = z + k / 3 - 1;
+    in[k] = yy >= 0 && zz >= 0 && yy < bm.by && zz < bm.bz;
+    const int w = in[k] ? (zz * bm.by + yy) * wpr + wx : t;
+    o[k] = obs[w];
+    o_lo[k] = obs[wx > 0 ? w - 1 : w];
+    o_hi[k] = obs[wx + 1 < wpr ? w + 1 : w];
+    p[k] = parts4[w];
+    p_lo[k] = parts[wx > 0 ? 4 * w - 1 : 4 * w];
+    p_hi[k] = parts[wx + 1 < wpr ? 4 * w + 4 : 4 * w];
+  }
+  uint32_t seen = 0U, seen_lo = 0U, seen_hi = 0U;  // lo / hi: the neighbouring words' edge blocks
+  uint32_t full = 0xFFFFFFFFU, full_lo = wx > 0 ? 1U : 0U, full_hi = wx + 1 < wpr ? 1U : 0U;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    if (!in[k]) continue;
+    seen |= o[k];
+    if (wx > 0) seen_lo |= o_lo[k] >> 31;
+    if (wx + 1 < wpr) seen_hi |= o_hi[k] & 1U;
+    full &= full_blocks(p[k]);
+    full_lo &= (p_lo[k] >> 28) == 0xFU ? 1U : 0U;
+    full_hi &= (p_hi[k] & 0xFU) == 0xFU ? 1U : 0U;
+  }
+  skip[t] = ~(seen | (seen << 1) | (seen >> 1) | seen_lo | (seen_hi << 31));
+  free_bits[t] = interior ? full & ((full << 1) | full_lo) & ((full >> 1) | (full_hi << 31)) : 0U;
 }
 
 // observed bits from the voxel array (after an upload): one thread per block.
@@ -968,9 +1083,16 @@ void launch_fill_voxels(kf_context* ctx) {
 void launch_skip_map(kf_context* ctx, bool end_frame) {
   const BlockMap bm = block_map(ctx->grid);
   const int nblocks = bm.bx * bm.by * bm.bz;
-  hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
-                     ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks, end_frame ? ctx->icp : nullptr,
-                     ctx->trajectory);
+  if (bm.bx % 32 == 0) {
+    const int words = nblocks / 32;
+    hipLaunchKernelGGL(skip_map_rows_kernel, dim3((words + 63) / 64), dim3(64), 0, ctx->stream, ctx->obs_bits,
+                       ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, end_frame ? ctx->icp : nullptr,
+                       ctx->trajectory);
+  } else {
+    hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
+                       ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks, end_frame ? ctx->icp : nullptr,
+                       ctx->trajectory);
+  }
   KF_HIP(hipGetLastError());
 }
 
@@ -1057,12 +1179,15 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
     KF_HIP(hipGetLastError());
     return;
   }
-  hipLaunchKernelGGL(tile_pyramid_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->tile_max, tiles_x, tiles_y);
   const Grid& g = ctx->grid;
   const int nb = ((g.nx + kBrickX - 1) / kBrickX) * ((g.ny + kBrickY - 1) / kBrickY) *
                  ((g.nz + kBrickZ - 1) / kBrickZ);
-  hipLaunchKernelGGL(brick_cull_kernel, dim3((nb + 255) / 256), dim3(256), 0, ctx->stream, g, f,
-                     ctx->tile_max, tiles_x, tiles_y, ctx->bricks, ctx->counters);
+  const int t1x = (tiles_x + 3) / 4, t1y = (tiles_y + 3) / 4;
+  const size_t level_bytes = (static_cast<size_t>(t1x) * t1y + ((t1x + 3) / 4) * ((t1y + 3) / 4)) * sizeof(uint32_t);
+  const bool group_levels = level_bytes <= static_cast<size_t>(kPyramidLdsBytes);
+  hipLaunchKernelGGL(brick_cull_kernel, dim3((nb + 256 * kCullBricks - 1) / (256 * kCullBricks)), dim3(256),
+                     group_levels ? level_bytes : 0, ctx->stream, g, f, ctx->tile_max, tiles_x, tiles_y, ctx->bricks,
+                     ctx->counters, group_levels ? 1 : 0);
   if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
   switch (c.voxel_store) {
     case KF_VOXEL_QUANTIZED:

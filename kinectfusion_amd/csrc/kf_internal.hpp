@@ -22,13 +22,15 @@
 namespace kf {
 
 constexpr int kMaxLevels = 4;
-constexpr int kBrickX = 64;  // integration brick: 64 x 4 x 8 voxels (a z-column of 8 per thread)
+// integration brick: 16 x 4 x 8 voxels, one wave (a z-column of 8 per lane);
+// it is two 8^3 blocks along x, one y-half of each
+constexpr int kBrickX = 16;
 constexpr int kBrickY = 4;
 constexpr int kBrickZ = 8;
-// A listed brick: its coordinates packed as bx | by << 8 | bz << 20
-// (resolution <= 4096 per axis: bx < 64, by < 1024, bz < 512).
+// A listed brick: its coordinates packed as bx | by << 8 | bz << 18
+// (resolution <= 4096 per axis: bx < 256, by < 1024, bz < 512).
 __host__ __device__ inline uint32_t pack_brick(int bx, int by, int bz) {
-  return static_cast<uint32_t>(bx) | (static_cast<uint32_t>(by) << 8U) | (static_cast<uint32_t>(bz) << 20U);
+  return static_cast<uint32_t>(bx) | (static_cast<uint32_t>(by) << 8U) | (static_cast<uint32_t>(bz) << 18U);
 }
 constexpr int kTile = 16;    // depth-tile edge for the integration cull
 

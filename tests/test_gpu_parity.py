@@ -442,6 +442,31 @@ def test_captured_frames_match_eager_frames(K, scene, torch_gpu):
     assert np.array_equal(eager[1], graph[1]) and np.array_equal(eager[2], graph[2])
 
 
+@pytest.mark.parametrize("res", [256, 200])
+def test_skip_map_is_the_unobserved_neighbourhood(K, scene, torch_gpu, res):
+    """The skip bit of a block is set exactly when no block among the 27
+    around it (inside the grid) is observed, and a free block has all 27
+    observed: at 256^3 a row of blocks fills whole bit words (the word-
+    parallel map kernel), at 200^3 it does not (the per-block kernel)."""
+    from scipy.ndimage import binary_dilation, binary_erosion
+
+    p = K.Pipeline(pipeline_config(K, res=res, store="bf16", voxel=2.56 / res))
+    for i in range(4):
+        depth, rgba, _ = scene.frame(i)
+        p.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba))
+    maps = p.block_bitmaps()
+    p.close()
+    nb = (res + 7) // 8
+    cube = np.ones((3, 3, 3), bool)
+    obs = maps["observed"].reshape(nb, nb, nb)
+    skip = maps["skip"].reshape(nb, nb, nb)
+    free = maps["free"].reshape(nb, nb, nb)
+    assert 0 < obs.sum() < obs.size
+    assert np.array_equal(skip, ~binary_dilation(obs, structure=cube))
+    assert not np.any(free & ~binary_erosion(obs, structure=cube, border_value=0))
+    print(f"{res}^3: observed {obs.sum()}, skip {skip.sum()}, free {free.sum()}")
+
+
 @pytest.mark.parametrize("store", ["float", "bf16"])
 def test_free_space_blocks_hold_only_ones(K, scene, torch_gpu, store):
     """A block the raycaster steps through as free space must hold, with its
