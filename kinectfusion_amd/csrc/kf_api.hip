@@ -67,8 +67,9 @@ void dfree(void* p) {
   if (p != nullptr) (void)hipFree(p);
 }
 
-__global__ void frame_begin_kernel(IcpState* st, int init_mode) {
-  if (threadIdx.x != 0) return;
+// A tracked frame's start (one thread): the frame marks are cleared and the
+// pose seeded from the previous pose, or by constant velocity.
+__device__ void frame_begin(IcpState* st, int init_mode) {
   st->frame_failed = 0;
   st->failed_level = -1;
   st->used_learned = 0;
@@ -92,6 +93,10 @@ __global__ void frame_begin_kernel(IcpState* st, int init_mode) {
   } else {
     for (int i = 0; i < 16; ++i) st->pose[i] = st->prev_pose[i];
   }
+}
+
+__global__ void frame_begin_kernel(IcpState* st, int init_mode) {
+  if (threadIdx.x == 0) frame_begin(st, init_mode);
 }
 
 
@@ -148,10 +153,11 @@ __device__ void rodrigues64(const double w[3], double R[9]) {
     }
 }
 
-// The learned seed: keep the constant-velocity seed (already in pose) for
-// the fallback, then pose = prev_pose [R(omega) | t].
+// The learned seed, at the frame's start: the constant-velocity seed is kept
+// for the fallback, then pose = prev_pose [R(omega) | t].
 __global__ void learned_seed_kernel(IcpState* st, const float* twist) {
   if (threadIdx.x != 0) return;
+  frame_begin(st, KF_INIT_CONSTANT_VELOCITY);
   const double w[3] = {twist[0], twist[1], twist[2]};
   double R[9];
   rodrigues64(w, R);
@@ -171,48 +177,6 @@ __global__ void learned_seed_kernel(IcpState* st, const float* twist) {
   }
   st->used_learned = 0;
   st->learned_rms = -1.0;
-}
-
-// After the run from the proposal: keep it when no level failed and the
-// final point-to-plane RMS is at most max_rms; otherwise restore the
-// constant-velocity seed for the full run.  Kept frames mark the frame
-// failed so the full run's kernels return at once (learned_end_kernel
-// clears the mark).
-__global__ void learned_check_kernel(IcpState* st, int finest, float max_rms) {
-  if (threadIdx.x != 0) return;
-  const kf_icp_result& r = st->results[finest];
-  const double rms = r.mean_squared_residual >= 0.0 ? sqrt(r.mean_squared_residual) : 1e300;
-  st->learned_rms = rms;
-  if (st->frame_failed == 0 && (r.status == 0 || r.status == 1) && rms <= static_cast<double>(max_rms)) {
-    st->used_learned = 1;
-    st->frame_failed = 1;
-  } else {
-    st->used_learned = 0;
-    st->frame_failed = 0;
-    st->failed_level = -1;
-    for (int i = 0; i < 16; ++i) st->pose[i] = st->seed_pose[i];
-  }
-}
-
-__global__ void learned_end_kernel(IcpState* st) {
-  if (threadIdx.x != 0) return;
-  if (st->used_learned != 0) {
-    st->frame_failed = 0;
-    st->failed_level = -1;
-  }
-  // the proposal's distance from the final pose: E = proposal^-1 pose
-  double inv[16];
-  rigid_inverse64(st->proposal, inv);
-  double E[16];
-  for (int r = 0; r < 4; ++r)
-    for (int c = 0; c < 4; ++c) {
-      double s = 0.0;
-      for (int q = 0; q < 4; ++q) s += inv[r * 4 + q] * st->pose[q * 4 + c];
-      E[r * 4 + c] = s;
-    }
-  st->proposal_err_t = sqrt(E[3] * E[3] + E[7] * E[7] + E[11] * E[11]);
-  const double ct = fmin(1.0, fmax(-1.0, 0.5 * (E[0] + E[5] + E[10] - 1.0)));
-  st->proposal_err_r = acos(ct);
 }
 
 template <int Kind>
@@ -333,17 +297,21 @@ void record(kf_context* ctx, int slot) {
 // Coarse-to-fine tracking from the device pose (track_pose).
 // Coarse-to-fine tracking with per-level iteration counts (0 skips a level
 // and its raycast).
+// check_max_rms > 0: the finest level run also makes the learned warm
+// start's check.
 void track_levels(kf_context* ctx, const DeviceIcpParams& p, bool timing, const int32_t* iterations,
-                  bool single_launch = false) {
+                  bool single_launch = false, float check_max_rms = 0.0F) {
   const kf_config& c = ctx->cfg;
   int slot = 2;
+  int finest = 0;
+  while (finest + 1 < c.levels && iterations[finest] == 0) ++finest;
   for (int l = c.levels - 1; l >= 0; --l) {
     LevelBuffers& L = ctx->lv[l];
     if (iterations[l] > 0) launch_raycast(ctx, l, nullptr, nullptr, true, l);  // + begin the level
     if (timing) record(ctx, slot++);
     if (iterations[l] > 0) {
       launch_icp_level(ctx, L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k, iterations[l], p,
-                       l, single_launch);
+                       l, single_launch, l == finest ? check_max_rms : 0.0F);
     }
     if (timing) record(ctx, slot++);
   }
@@ -380,20 +348,13 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     // learned warm start: propose, track from the proposal, fall back to the
     // constant-velocity seed with the full schedule when it does not hold
     const kf_config& c = ctx->cfg;
-    hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
-                       static_cast<int>(KF_INIT_CONSTANT_VELOCITY));
     posenet_propose(ctx->posenet, ctx->stream, ctx->prev_coarse, ctx->lv[c.levels - 1].depth, ctx->twist);
     hipLaunchKernelGGL(learned_seed_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->twist);
     record(ctx, 14);
     const DeviceIcpParams p = to_device_params(opt);
-    track_levels(ctx, p, timing, c.learned_iterations);
-    int finest = 0;
-    while (c.learned_iterations[finest] == 0) ++finest;
-    hipLaunchKernelGGL(learned_check_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, finest,
-                       c.learned_max_rms);
+    track_levels(ctx, p, timing, c.learned_iterations, false, c.learned_max_rms);  // + the check
     // the fallback pass runs only when the check failed: one launch per trial
     track_levels(ctx, p, false, c.icp_iterations, true);
-    hipLaunchKernelGGL(learned_end_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     record(ctx, 2 * c.levels + 1);  // the fallback run counts as tracking time
   } else {
     hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,
@@ -402,13 +363,11 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     track_levels(ctx, to_device_params(opt), timing, ctx->cfg.icp_iterations);
   }
   ctx->timed_learned = timing && !first && learned_active(ctx);
-  launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true, true);  // + the frame's end
+  // + the learned run's end (in the pixel pass) and the frame's end (in the
+  // skip-map pass, which also keeps this frame's coarse depth for the next
+  // proposal)
+  launch_integrate(ctx, frame, ctx->cfg.count_fused != 0, true, true, true, !first && learned_active(ctx));
   KF_HIP(hipGetLastError());
-  if (ctx->cfg.init_mode == KF_INIT_LEARNED && ctx->prev_coarse != nullptr) {
-    const LevelBuffers& L = ctx->lv[ctx->cfg.levels - 1];
-    KF_HIP(hipMemcpyAsync(ctx->prev_coarse, L.depth, sizeof(uint16_t) * L.w * L.h, hipMemcpyDeviceToDevice,
-                          ctx->stream));
-  }
 }
 
 void copy_results(const kf_context* ctx, kf_icp_result* results) {

@@ -451,6 +451,8 @@ __device__ void run_trial(IcpState* st, const DeviceIcpParams& p, const float* s
   if (accepted && ut < p.min_t && ang < p.min_r) st->status = 1;
 }
 
+__device__ void finish_level_block(IcpState* st, const DeviceIcpParams& p, int level, float check_rms);
+
 // One trial's correspondence sweep: one partial row per workgroup.
 __global__ void __launch_bounds__(256) icp_trial_sweep_kernel(Maps m, DeviceIcpParams p, IcpState* st,
                                                                float* __restrict__ partial) {
@@ -489,11 +491,15 @@ __device__ inline bool last_block(unsigned int* ticket) {
 // serves passes that usually exit at once (latched status).
 __global__ void __launch_bounds__(256) icp_trial_single_kernel(Maps m, DeviceIcpParams p, IcpState* st,
                                                                 float* __restrict__ partial,
-                                                                unsigned int* __restrict__ ticket) {
+                                                                unsigned int* __restrict__ ticket, int finish_level) {
   __shared__ float lds[4][kSums + 1];
   __shared__ float combined[kSums + 1];
   __shared__ float xf[24];
-  if (st->status != 0 || st->frame_failed) return;  // latched: nothing to do
+  if (st->frame_failed) return;
+  if (st->status != 0) {  // latched: nothing to do but the level's finish
+    if (finish_level >= 0 && blockIdx.x == 0) finish_level_block(st, p, finish_level, 0.0F);
+    return;
+  }
   if (threadIdx.x == 0) {
     double inv[16];
     rigid_inverse64(st->model_pose, inv);
@@ -513,6 +519,10 @@ __global__ void __launch_bounds__(256) icp_trial_single_kernel(Maps m, DeviceIcp
     *ticket = 0U;
     run_trial(st, p, combined);
   }
+  if (finish_level >= 0) {
+    __syncthreads();
+    finish_level_block(st, p, finish_level, 0.0F);
+  }
 }
 
 // The trial's reduction (one workgroup: the partial rows into st->sums32)
@@ -520,13 +530,19 @@ __global__ void __launch_bounds__(256) icp_trial_single_kernel(Maps m, DeviceIcp
 // the partial rows of every sweep workgroup are visible without a
 // device-scope fence in each of them.
 __global__ void __launch_bounds__(256) icp_trial_solve_kernel(DeviceIcpParams p, IcpState* st,
-                                                               const float* __restrict__ partial, int nblocks) {
+                                                               const float* __restrict__ partial, int nblocks,
+                                                               int finish_level, float check_rms) {
   __shared__ float combined[kSums + 1];
-  if (st->status != 0 || st->frame_failed) return;
-  combine_partials(partial, nblocks, combined);
-  __syncthreads();
-  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
-  if (threadIdx.x == 0) run_trial(st, p, combined);
+  if (st->status == 0 && st->frame_failed == 0) {
+    combine_partials(partial, nblocks, combined);
+    __syncthreads();
+    if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
+    if (threadIdx.x == 0) run_trial(st, p, combined);
+  }
+  if (finish_level >= 0) {
+    __syncthreads();
+    finish_level_block(st, p, finish_level, check_rms);
+  }
 }
 
 // Single sweep with explicit float transforms (kf_icp_reduce), then its
@@ -671,40 +687,55 @@ __global__ void __launch_bounds__(64) icp_begin_level_kernel(IcpState* st, int l
   if (threadIdx.x == 0) icp_begin_level(st, level);
 }
 
-// Stability check and result record (icp_optimizer.cpp:108-169).
-__global__ void __launch_bounds__(64) icp_finish_level_kernel(IcpState* st, DeviceIcpParams p, int level) {
+// Stability check and result record of a level (icp_optimizer.cpp:108-169),
+// by a whole workgroup (the Jacobi sweeps take its first wave), then, when
+// check_rms > 0, the learned warm start's check of this level.
+__device__ void finish_level_block(IcpState* st, const DeviceIcpParams& p, int level, float check_rms) {
   __shared__ double ev[6];
-  if (st->frame_failed) return;
-  const int status0 = st->status;
-  const bool check = (status0 == 0 || status0 == 1) && st->eq_count >= p.min_corr;
-  if (check) sym_eig6_wave(st->eq, ev);  // the whole wave
-  if (threadIdx.x != 0) return;
-  kf_icp_result& r = st->results[level];
-  int status = status0;
-  r.min_eigenvalue = 0.0;
-  r.condition_number = 0.0;
-  if (status == 0 || status == 1) {
-    const bool stable = check && stable_from_eigenvalues(ev, p, &r.min_eigenvalue, &r.condition_number);
-    if (!stable && p.damping_mode == 0) status = 5;
+  __shared__ int skip, check;
+  if (threadIdx.x == 0) {
+    skip = st->frame_failed;
+    check = (st->status == 0 || st->status == 1) && st->eq_count >= p.min_corr ? 1 : 0;
   }
-  r.status = status;
-  r.trials = st->trials;
-  r.rejected = st->rejected;
-  r.correspondences = st->eq_count;
-  if (st->eq_count > 0) {
-    r.mean_distance = st->eq[27] / static_cast<double>(st->eq_count);
-    r.mean_squared_residual = st->eq[28] / static_cast<double>(st->eq_count);
+  __syncthreads();
+  if (skip == 0) {
+    if (check != 0) sym_eig6_wave(st->eq, ev);
+    if (threadIdx.x == 0) {
+      kf_icp_result& r = st->results[level];
+      int status = st->status;
+      r.min_eigenvalue = 0.0;
+      r.condition_number = 0.0;
+      if (status == 0 || status == 1) {
+        const bool stable = check != 0 && stable_from_eigenvalues(ev, p, &r.min_eigenvalue, &r.condition_number);
+        if (!stable && p.damping_mode == 0) status = 5;
+      }
+      r.status = status;
+      r.trials = st->trials;
+      r.rejected = st->rejected;
+      r.correspondences = st->eq_count;
+      if (st->eq_count > 0) {
+        r.mean_distance = st->eq[27] / static_cast<double>(st->eq_count);
+        r.mean_squared_residual = st->eq[28] / static_cast<double>(st->eq_count);
+      }
+      r.update_translation = st->update_t;
+      r.update_rotation = st->update_r;
+      r.lambda = st->lambda;
+      st->status = status;
+      if (status >= 2) {
+        // failure: the outcome keeps the level's initial pose
+        for (int i = 0; i < 16; ++i) st->pose[i] = st->level_pose0[i];
+        st->frame_failed = 1;
+        st->failed_level = level;
+      }
+    }
   }
-  r.update_translation = st->update_t;
-  r.update_rotation = st->update_r;
-  r.lambda = st->lambda;
-  st->status = status;
-  if (status >= 2) {
-    // failure: the outcome keeps the level's initial pose
-    for (int i = 0; i < 16; ++i) st->pose[i] = st->level_pose0[i];
-    st->frame_failed = 1;
-    st->failed_level = level;
-  }
+  if (check_rms > 0.0F && threadIdx.x == 0) learned_check(st, level, check_rms);
+}
+
+// A level with no trial: its finish alone.
+__global__ void __launch_bounds__(64) icp_finish_level_kernel(IcpState* st, DeviceIcpParams p, int level,
+                                                               float check_rms) {
+  finish_level_block(st, p, level, check_rms);
 }
 
 Maps make_maps(const float4* lv_v, const float4* lv_n, int lw, int lh, const float4* md_v,
@@ -744,20 +775,29 @@ void launch_icp_begin_level(kf_context* ctx, int level) {
 
 void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                       const float4* md_v, const float4* md_n, int mw, int mh, Intr k, int iterations,
-                      const DeviceIcpParams& p, int level, bool single_launch) {
+                      const DeviceIcpParams& p, int level, bool single_launch, float check_max_rms) {
   const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
   const int nb = blocks_for(lw * lh);
+  if (iterations <= 0) {
+    hipLaunchKernelGGL(icp_finish_level_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, p, level,
+                       check_max_rms);
+    KF_HIP(hipGetLastError());
+    return;
+  }
+  // the level's last trial also finishes the level (and runs the learned check)
   for (int it = 0; it < iterations; ++it) {
+    const int finish = it + 1 == iterations ? level : -1;
     if (single_launch) {
+      if (check_max_rms > 0.0F) fail(KF_ERR_INVALID_ARGUMENT, "icp level: the learned check needs two-kernel trials");
       hipLaunchKernelGGL(icp_trial_single_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
-                         ctx->partials, ctx->ticket);
+                         ctx->partials, ctx->ticket, finish);
       continue;
     }
     hipLaunchKernelGGL(icp_trial_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
                        ctx->partials);
-    hipLaunchKernelGGL(icp_trial_solve_kernel, dim3(1), dim3(256), 0, ctx->stream, p, ctx->icp, ctx->partials, nb);
+    hipLaunchKernelGGL(icp_trial_solve_kernel, dim3(1), dim3(256), 0, ctx->stream, p, ctx->icp, ctx->partials, nb,
+                       finish, finish >= 0 ? check_max_rms : 0.0F);
   }
-  hipLaunchKernelGGL(icp_finish_level_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, p, level);
   KF_HIP(hipGetLastError());
 }
 

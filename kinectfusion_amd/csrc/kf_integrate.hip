@@ -63,12 +63,15 @@ __global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restr
                                                           float* __restrict__ tile_max,
                                                           FrameParams f, float base_trunc,
                                                           uint32_t* __restrict__ counters,
-                                                          unsigned long long* __restrict__ fused) {
+                                                          unsigned long long* __restrict__ fused,
+                                                          IcpState* learned_st) {
   __shared__ float red[4];
-  // the brick and fused counters start the pass at zero (read by later launches only)
+  // the brick and fused counters start the pass at zero (read by later
+  // launches only); learned_st: the learned warm start's end of the frame
   if (blockIdx.x == 0 && blockIdx.y == 0) {
     if (threadIdx.y == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0U;
     if (threadIdx.y == 0 && threadIdx.x < 2) fused[threadIdx.x] = 0ULL;
+    if (learned_st != nullptr && threadIdx.y == 0 && threadIdx.x == 0) learned_end(learned_st);
   }
   const int col = blockIdx.x * kTile + threadIdx.x, row = blockIdx.y * kTile + threadIdx.y;
   const int tid = threadIdx.y * kTile + threadIdx.x;
@@ -594,6 +597,18 @@ __device__ inline bool block_free(const uint32_t* __restrict__ parts, int fb) {
   return ((parts[fb >> 3] >> ((fb & 7) * 4)) & 0xFU) == 0xFU;
 }
 
+// A copy of n 16-bit words made by a whole grid on the side of its own work
+// (n = 0: none).
+struct CopyU16 {
+  const uint16_t* src;
+  uint16_t* dst;
+  int n;
+  __device__ inline void run() const {
+    const int stride = static_cast<int>(gridDim.x * blockDim.x);
+    for (int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x); i < n; i += stride) dst[i] = src[i];
+  }
+};
+
 // The end of a frame: its pose goes to the trajectory slot of the device
 // frame counter and becomes the previous pose.
 __device__ inline void close_frame(IcpState* st, double* trajectory) {
@@ -611,7 +626,8 @@ __device__ inline void close_frame(IcpState* st, double* trajectory) {
 __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restrict__ obs, uint32_t* __restrict__ skip,
                                                         const uint32_t* __restrict__ parts,
                                                         uint32_t* __restrict__ free_bits, BlockMap bm, int nblocks,
-                                                        IcpState* end_state, double* trajectory) {
+                                                        IcpState* end_state, double* trajectory, CopyU16 keep) {
+  keep.run();
   // end_state: this launch also closes the frame (one thread): the
   // trajectory slot is the device frame counter, so the same launch serves
   // every frame of a captured graph
@@ -686,8 +702,9 @@ __global__ void __launch_bounds__(64) skip_map_rows_kernel(const uint32_t* __res
                                                              uint32_t* __restrict__ skip,
                                                              const uint32_t* __restrict__ parts,
                                                              uint32_t* __restrict__ free_bits, BlockMap bm,
-                                                             IcpState* end_state, double* trajectory) {
+                                                             IcpState* end_state, double* trajectory, CopyU16 keep) {
   if (end_state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) close_frame(end_state, trajectory);
+  keep.run();
   const int wpr = bm.bx >> 5;  // words per row
   const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
   if (t >= wpr * bm.by * bm.bz) return;
@@ -1083,15 +1100,21 @@ void launch_fill_voxels(kf_context* ctx) {
 void launch_skip_map(kf_context* ctx, bool end_frame) {
   const BlockMap bm = block_map(ctx->grid);
   const int nblocks = bm.bx * bm.by * bm.bz;
+  // the frame's end keeps its coarse depth for the next frame's proposal
+  CopyU16 keep{nullptr, nullptr, 0};
+  if (end_frame && ctx->cfg.init_mode == KF_INIT_LEARNED && ctx->prev_coarse != nullptr) {
+    const LevelBuffers& L = ctx->lv[ctx->cfg.levels - 1];
+    keep = CopyU16{L.depth, ctx->prev_coarse, L.w * L.h};
+  }
   if (bm.bx % 32 == 0) {
     const int words = nblocks / 32;
     hipLaunchKernelGGL(skip_map_rows_kernel, dim3((words + 63) / 64), dim3(64), 0, ctx->stream, ctx->obs_bits,
                        ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, end_frame ? ctx->icp : nullptr,
-                       ctx->trajectory);
+                       ctx->trajectory, keep);
   } else {
     hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
                        ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks, end_frame ? ctx->icp : nullptr,
-                       ctx->trajectory);
+                       ctx->trajectory, keep);
   }
   KF_HIP(hipGetLastError());
 }
@@ -1119,7 +1142,7 @@ void launch_rebuild_observed(kf_context* ctx) {
 }
 
 void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, bool from_state, bool end_frame,
-                      bool accumulate) {
+                      bool accumulate, bool end_learned) {
   const kf_config& c = ctx->cfg;
   FrameParams f{};
   f.w2c = rigid_from12(fr.world_to_camera);
@@ -1142,7 +1165,7 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
     fail(KF_ERR_INVALID_ARGUMENT, "integrate: frame larger than the configured camera");
   hipLaunchKernelGGL(pixel_prep_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream,
                      fr.depth, reinterpret_cast<const float4*>(fr.normals), ctx->pix, ctx->tile_max,
-                     f, ctx->grid.trunc, ctx->counters, ctx->fused);
+                     f, ctx->grid.trunc, ctx->counters, ctx->fused, end_learned ? ctx->icp : nullptr);
   const bool color = ctx->colors != nullptr;
   const uint32_t* cimg = fr.color;
   const bool timing = c.stage_timing != 0 && from_state;

@@ -321,6 +321,48 @@ __device__ inline void icp_begin_level(IcpState* st, int level) {
   }
 }
 
+// The learned warm start's check after its run (one thread): keep the run
+// when no level failed and the finest level's point-to-plane RMS is at most
+// max_rms, marking the frame failed so that the fallback pass exits at once
+// (the frame's end clears the mark); otherwise restore the constant-velocity
+// seed for the fallback pass.
+__device__ inline void learned_check(IcpState* st, int finest, float max_rms) {
+  const kf_icp_result& r = st->results[finest];
+  const double rms = r.mean_squared_residual >= 0.0 ? sqrt(r.mean_squared_residual) : 1e300;
+  st->learned_rms = rms;
+  if (st->frame_failed == 0 && (r.status == 0 || r.status == 1) && rms <= static_cast<double>(max_rms)) {
+    st->used_learned = 1;
+    st->frame_failed = 1;
+  } else {
+    st->used_learned = 0;
+    st->frame_failed = 0;
+    st->failed_level = -1;
+    for (int i = 0; i < 16; ++i) st->pose[i] = st->seed_pose[i];
+  }
+}
+
+// The learned warm start's end of a frame (one thread): a kept run's frame
+// mark is cleared, and the proposal's distance from the final pose recorded.
+__device__ inline void learned_end(IcpState* st) {
+  if (st->used_learned != 0) {
+    st->frame_failed = 0;
+    st->failed_level = -1;
+  }
+  // the proposal's distance from the final pose: E = proposal^-1 pose
+  double inv[16];
+  rigid_inverse64(st->proposal, inv);
+  double E[16];
+  for (int r = 0; r < 4; ++r)
+    for (int c = 0; c < 4; ++c) {
+      double s = 0.0;
+      for (int q = 0; q < 4; ++q) s += inv[r * 4 + q] * st->pose[q * 4 + c];
+      E[r * 4 + c] = s;
+    }
+  st->proposal_err_t = sqrt(E[3] * E[3] + E[7] * E[7] + E[11] * E[11]);
+  const double ct = fmin(1.0, fmax(-1.0, 0.5 * (E[0] + E[5] + E[10] - 1.0)));
+  st->proposal_err_r = acos(ct);
+}
+
 struct DeviceIcpParams {
   long long min_corr;
   float max_dist, min_dot;
@@ -430,8 +472,9 @@ void hip_check(hipError_t err, const char* what);
 // Launch helpers implemented per stage.
 void launch_bilateral(kf_context* ctx, const uint16_t* raw);
 void launch_level_maps(kf_context* ctx, int level);
+// end_learned: the pixel pass also ends the learned warm start's frame.
 void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state,
-                      bool end_frame = false, bool accumulate = false);
+                      bool end_frame = false, bool accumulate = false, bool end_learned = false);
 // begin_level >= 0: the launch also prepares that tracking level
 // (icp_begin_level, by one thread) -- one launch fewer per level.
 void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
@@ -442,7 +485,8 @@ void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, 
 // start's fallback); otherwise two kernels per trial.
 void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                       const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
-                      int iterations, const DeviceIcpParams& p, int level_index, bool single_launch = false);
+                      int iterations, const DeviceIcpParams& p, int level_index, bool single_launch = false,
+                      float check_max_rms = 0.0F);
 void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                             const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
                             const float* camera12, const float* model12, float max_dist,
