@@ -608,78 +608,72 @@ __device__ void sym_eig6(const double* eq, double* ev) {
 }
 
 // The same eigenvalues computed by one wave (the stability check at the end
-// of a level, where 64 threads are at hand).  Each sweep is 5 rounds of 3
-// rotations on disjoint index pairs (all 15 pairs once, round-robin order):
-// three lanes form the rotations of a round side by side, then 36 lanes
-// apply them to one element each, columns first, then rows.  Rounding
-// differs from the cyclic single-thread order; the eigenvalues agree to
-// ~1e-15 of the norm.
+// of a level).  Each sweep is 5 rounds of 3 rotations on disjoint index
+// pairs (all 15 pairs once, round-robin order).  Lane r * 6 + c (< 36) holds
+// A[r][c]; each lane forms the rotations of its row and its column itself
+// and reads the three other entries of J^T A J it needs by shuffles, with
+// the arithmetic of B = A J, then J^T B, entry by entry; no barriers.
+// Rounding differs from the cyclic single-thread order; the eigenvalues
+// agree to ~1e-15 of the norm.
 namespace jacobi {
 constexpr int kP[5][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 1}, {0, 1, 4}, {0, 2, 3}};
 constexpr int kQ[5][3] = {{5, 4, 3}, {4, 5, 2}, {3, 4, 5}, {2, 3, 5}, {1, 5, 4}};
 }  // namespace jacobi
 
 __device__ void sym_eig6_wave(const double* eq, double* ev) {
-  __shared__ double A[6][6], B[6][6];
-  __shared__ double rc[3], rs[3];
-  __shared__ int converged;
-  const int lane = threadIdx.x;
-  const int r = lane / 6, c = lane % 6;
-  if (lane < 36) {
-    const int lo = min(r, c), hi = max(r, c);
-    A[r][c] = eq[lo * 6 - (lo * (lo - 1)) / 2 + (hi - lo)];  // packed upper triangle, row by row
-  }
-  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const bool on = lane < 36;
+  const int r = on ? lane / 6 : 0, c = on ? lane % 6 : 0;
+  const int lo = min(r, c), hi = max(r, c);
+  double a = on ? eq[lo * 6 - (lo * (lo - 1)) / 2 + (hi - lo)] : 0.0;  // packed upper triangle, row by row
   for (int sweep = 0; sweep < 32; ++sweep) {
-    if (lane == 0) {
-      double off = 0.0, diag = 0.0;
-      for (int d = 0; d < 6; ++d) diag += A[d][d] * A[d][d];
-      for (int pp = 0; pp < 6; ++pp)
-        for (int q = pp + 1; q < 6; ++q) off += A[pp][q] * A[pp][q];
-      converged = off <= 1.0e-32 * diag ? 1 : 0;
+    double off = on && r < c ? a * a : 0.0, diag = on && r == c ? a * a : 0.0;
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o);
+      diag += __shfl_xor(diag, o);
     }
-    __syncthreads();
-    if (converged != 0) break;
+    if (off <= 1.0e-32 * diag) break;
     for (int round = 0; round < 5; ++round) {
-      if (lane < 3) {
-        const int pp = jacobi::kP[round][lane], q = jacobi::kQ[round][lane];
-        double cc = 1.0, ss = 0.0;
-        if (A[pp][q] != 0.0) {
-          const double theta = (A[q][q] - A[pp][pp]) / (2.0 * A[pp][q]);
+      // the rotation holding index j: its partner, whether j is its first index
+      auto find = [&](int j, int& other, bool& first) {
+        other = 0;
+        first = true;
+        for (int i = 0; i < 3; ++i) {
+          if (jacobi::kP[round][i] == j) { other = jacobi::kQ[round][i]; first = true; }
+          if (jacobi::kQ[round][i] == j) { other = jacobi::kP[round][i]; first = false; }
+        }
+      };
+      int orow, ocol;
+      bool frow, fcol;
+      find(r, orow, frow);
+      find(c, ocol, fcol);
+      const int pr = frow ? r : orow, qr = frow ? orow : r, pc = fcol ? c : ocol, qc = fcol ? ocol : c;
+      const double app_r = __shfl(a, pr * 7), aqq_r = __shfl(a, qr * 7), apq_r = __shfl(a, pr * 6 + qr);
+      const double app_c = __shfl(a, pc * 7), aqq_c = __shfl(a, qc * 7), apq_c = __shfl(a, pc * 6 + qc);
+      const double a_r_oc = __shfl(a, r * 6 + ocol), a_or_c = __shfl(a, orow * 6 + c),
+                   a_or_oc = __shfl(a, orow * 6 + ocol);
+      auto rotation = [](double app, double aqq, double apq, double& cc, double& ss) {
+        cc = 1.0;
+        ss = 0.0;
+        if (apq != 0.0) {
+          const double theta = (aqq - app) / (2.0 * apq);
           const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
           cc = 1.0 / sqrt(t * t + 1.0);
           ss = t * cc;
         }
-        rc[lane] = cc;
-        rs[lane] = ss;
-      }
-      __syncthreads();
-      // the rotation holding index j, and whether j is its first index
-      auto find = [&](int j, int& k, int& other, bool& first) {
-        k = 0;
-        for (int i = 0; i < 3; ++i) {
-          if (jacobi::kP[round][i] == j) { k = i; other = jacobi::kQ[round][i]; first = true; }
-          if (jacobi::kQ[round][i] == j) { k = i; other = jacobi::kP[round][i]; first = false; }
-        }
       };
-      if (lane < 36) {  // columns: B = A J
-        int k, o;
-        bool first;
-        find(c, k, o, first);
-        B[r][c] = first ? rc[k] * A[r][c] - rs[k] * A[r][o] : rs[k] * A[r][o] + rc[k] * A[r][c];
-      }
-      __syncthreads();
-      if (lane < 36) {  // rows: A = J^T B
-        int k, o;
-        bool first;
-        find(r, k, o, first);
-        A[r][c] = first ? rc[k] * B[r][c] - rs[k] * B[o][c] : rs[k] * B[o][c] + rc[k] * B[r][c];
-      }
-      __syncthreads();
+      double cr, sr, cc, sc;
+      rotation(app_r, aqq_r, apq_r, cr, sr);
+      rotation(app_c, aqq_c, apq_c, cc, sc);
+      // B = A J at (r, c) and (orow, c), then A = J^T B at (r, c)
+      const double b_r = fcol ? cc * a - sc * a_r_oc : sc * a_r_oc + cc * a;
+      const double b_o = fcol ? cc * a_or_c - sc * a_or_oc : sc * a_or_oc + cc * a_or_c;
+      a = frow ? cr * b_r - sr * b_o : sr * b_o + cr * b_r;
     }
   }
-  if (lane < 6) ev[lane] = A[lane][lane];
-  __syncthreads();
+  const double d = __shfl(a, (lane < 6 ? lane : 0) * 7);
+  if (lane < 6) ev[lane] = d;
+  __builtin_amdgcn_wave_barrier();
 }
 
 // Prepare a level: status cleared, model pose = running pose.
@@ -699,7 +693,7 @@ __device__ void finish_level_block(IcpState* st, const DeviceIcpParams& p, int l
   }
   __syncthreads();
   if (skip == 0) {
-    if (check != 0) sym_eig6_wave(st->eq, ev);
+    if (check != 0 && threadIdx.x < 64) sym_eig6_wave(st->eq, ev);  // the first wave
     if (threadIdx.x == 0) {
       kf_icp_result& r = st->results[level];
       int status = st->status;
