@@ -5,10 +5,13 @@
 // Layout: one 32 x 8 output tile per 256-thread workgroup (four wave64 rows
 // of 32 x 2).  Each workgroup stages its input tile plus halo in LDS once and
 // every tap of the stencil reads LDS instead of global memory.
+#include "kf_exp.hpp"
 #include "kf_internal.hpp"
 
 namespace kf {
 namespace {
+
+__constant__ double kExp2J64[64] = KF_EXP2_J64_TABLE;
 
 constexpr int kTx = 32;
 constexpr int kTy = 8;
@@ -17,17 +20,20 @@ constexpr int kMaxRadius = 8;
 // Bilateral filter (BilateralFilter::operator(), depth_processing.hpp:70-114):
 // clamped (not mirrored) window, samples outside [min, max] and zeros ignored,
 // weight exp(d^2 * ss + dm^2 * sr) evaluated as one exponential of the float
-// exponent.  The exponential is taken in FP64 and rounded to float, which
-// matches a correctly rounded expf.
+// exponent.  The exponential is taken in FP64 (Tang's table-driven method,
+// relative error below 4e-16, the table staged in LDS) and rounded to
+// float, which matches a correctly rounded expf.
 __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restrict__ in,
                                                          uint16_t* __restrict__ out, int w, int h,
                                                          float scale, float lo, float hi, int radius,
                                                          float ss, float sr) {
   __shared__ float meters[(kTy + 2 * kMaxRadius) * (kTx + 2 * kMaxRadius)];
   __shared__ float raws[(kTy + 2 * kMaxRadius) * (kTx + 2 * kMaxRadius)];
+  __shared__ double exp_table[64];
   const int tw = kTx + 2 * radius, th = kTy + 2 * radius;
   const int x0 = blockIdx.x * kTx - radius, y0 = blockIdx.y * kTy - radius;
   const int tid = threadIdx.y * kTx + threadIdx.x;
+  if (tid < 64) exp_table[tid] = kExp2J64[tid];
   for (int i = tid; i < tw * th; i += kTx * kTy) {
     const int lx = i % tw, ly = i / tw;
     const int gx = x0 + lx, gy = y0 + ly;
@@ -62,7 +68,7 @@ __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restri
             static_cast<float>(((x - col) * (x - col)) + ((y - row) * (y - row)));
         const float dd = sm - cm;
         const float e = (pd2 * ss) + (dd * dd * sr);
-        const float wt = static_cast<float>(exp(static_cast<double>(e)));
+        const float wt = static_cast<float>(exp_nonpositive(static_cast<double>(e), exp_table));
         wsum_v += wt * raws[base + x];
         wsum += wt;
       }
