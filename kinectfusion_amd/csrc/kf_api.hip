@@ -353,7 +353,7 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     record(ctx, 14);
     const DeviceIcpParams p = to_device_params(opt);
     track_levels(ctx, p, timing, c.learned_iterations, false, c.learned_max_rms);  // + the check
-    // the fallback pass runs only when the check failed: one launch per trial
+    // the fallback pass runs only when the check failed: one launch per level
     track_levels(ctx, p, false, c.icp_iterations, true);
     record(ctx, 2 * c.levels + 1);  // the fallback run counts as tracking time
   } else {
@@ -545,8 +545,8 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->fused = dalloc<unsigned long long>(4);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     ctx->partials = dalloc<float>(512 * 32);
-    ctx->ticket = dalloc<unsigned int>(1);
-    KF_HIP(hipMemsetAsync(ctx->ticket, 0, sizeof(unsigned int), ctx->stream));
+    ctx->ticket = dalloc<unsigned int>(3);  // ticket, epoch, timeout flag of the one-launch ICP level
+    KF_HIP(hipMemsetAsync(ctx->ticket, 0, 3 * sizeof(unsigned int), ctx->stream));
     ctx->icp = dalloc<IcpState>(1);
     KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
     KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));

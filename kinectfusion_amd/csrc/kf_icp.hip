@@ -23,8 +23,9 @@
 //    FP64, convergence and failure latches.
 // Once a level's status latches (converged or failed) the remaining trial
 // kernels return at once, so a converged level costs no further sweeps.  The
-// stability spectrum (FP64 cyclic Jacobi) is checked once per level in
-// icp_finish_level_kernel, as in the reference's device loop.
+// stability spectrum (FP64 Jacobi) is checked once per level, by the level's
+// last trial, as in the reference's device loop.  A pass that usually exits
+// at once runs each level in one launch instead (icp_level_kernel).
 #include "kf_internal.hpp"
 
 namespace kf {
@@ -485,44 +486,77 @@ __device__ inline bool last_block(unsigned int* ticket) {
   return is_last;
 }
 
-// One trial in a single launch: the sweep, then in the last workgroup to
-// finish the reduction and the FP64 solve.  The fences cost more than the
-// extra launch of the two-kernel form when the trial runs, so this form
-// serves passes that usually exit at once (latched status).
-__global__ void __launch_bounds__(256) icp_trial_single_kernel(Maps m, DeviceIcpParams p, IcpState* st,
-                                                                float* __restrict__ partial,
-                                                                unsigned int* __restrict__ ticket, int finish_level) {
+// A whole level in one launch, for a pass whose levels usually exit at once
+// (the learned warm start's fallback, when the learned run was kept): the
+// workgroups run the level's trials in step.  Each trial: every workgroup
+// sweeps its pixels into its partial row; the last to finish (ticket
+// protocol) reduces them, solves, and publishes the trial by advancing an
+// epoch word with a device-scope release; the others wait for the epoch
+// with a device-scope acquire.  The grid is at most kMaxBlocks workgroups,
+// all resident at once (checked on the host).  A wait that outlasts
+// kSyncTimeoutTicks of the 100 MHz wall clock (far beyond any trial) gives
+// up: the workgroup raises sync[2] and exits, so every wave ends.
+// sync: [0] ticket, [1] epoch (only ever advanced: a launch reads its base
+// before any workgroup can advance it), [2] timeout flag.
+constexpr long long kSyncTimeoutTicks = 2000000;  // 20 ms
+
+__global__ void __launch_bounds__(256) icp_level_kernel(Maps m, DeviceIcpParams p, IcpState* st,
+                                                        float* __restrict__ partial, unsigned int* sync,
+                                                        int iterations, int level) {
   __shared__ float lds[4][kSums + 1];
   __shared__ float combined[kSums + 1];
   __shared__ float xf[24];
+  __shared__ int go, timed_out;
   if (st->frame_failed) return;
-  if (st->status != 0) {  // latched: nothing to do but the level's finish
-    if (finish_level >= 0 && blockIdx.x == 0) finish_level_block(st, p, finish_level, 0.0F);
-    return;
-  }
-  if (threadIdx.x == 0) {
-    double inv[16];
-    rigid_inverse64(st->model_pose, inv);
-    for (int i = 0; i < 12; ++i) {
-      xf[i] = static_cast<float>(st->pose[i]);
-      xf[12 + i] = static_cast<float>(inv[i]);
+  const unsigned int base = __hip_atomic_load(&sync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  for (int it = 0; it < iterations; ++it) {
+    if (threadIdx.x == 0) {
+      go = st->status == 0 ? 1 : 0;  // latched: the level is done
+      if (go != 0) {
+        double inv[16];
+        rigid_inverse64(st->model_pose, inv);
+        for (int i = 0; i < 12; ++i) {
+          xf[i] = static_cast<float>(st->pose[i]);
+          xf[12 + i] = static_cast<float>(inv[i]);
+        }
+      }
+    }
+    __syncthreads();
+    if (go == 0) break;
+    const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
+    sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
+    const unsigned int target = base + static_cast<unsigned int>(it) + 1U;
+    if (last_block(&sync[0])) {
+      combine_partials(partial, gridDim.x, combined);
+      __syncthreads();
+      if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
+      if (threadIdx.x == 0) {
+        sync[0] = 0U;
+        run_trial(st, p, combined);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __hip_atomic_store(&sync[1], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+    } else {
+      if (threadIdx.x == 0) {
+        timed_out = 0;
+        const long long t0 = wall_clock64();
+        while (static_cast<int>(__hip_atomic_load(&sync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) <
+               0) {
+          if (wall_clock64() - t0 > kSyncTimeoutTicks) {
+            timed_out = 1;
+            atomicOr(&sync[2], 1U);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(2);
+        }
+      }
+      __syncthreads();
+      if (timed_out != 0) return;
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
   }
-  __syncthreads();
-  const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
-  sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
-  if (!last_block(ticket)) return;
-  combine_partials(partial, gridDim.x, combined);
-  __syncthreads();
-  if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
-  if (threadIdx.x == 0) {
-    *ticket = 0U;
-    run_trial(st, p, combined);
-  }
-  if (finish_level >= 0) {
-    __syncthreads();
-    finish_level_block(st, p, finish_level, 0.0F);
-  }
+  if (blockIdx.x == 0) finish_level_block(st, p, level, 0.0F);
 }
 
 // The trial's reduction (one workgroup: the partial rows into st->sums32)
@@ -778,15 +812,26 @@ void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, i
     KF_HIP(hipGetLastError());
     return;
   }
+  if (single_launch) {
+    if (check_max_rms > 0.0F) fail(KF_ERR_INVALID_ARGUMENT, "icp level: the learned check needs two-kernel trials");
+    static int resident = -1;  // workgroups of icp_level_kernel resident at once on this device
+    if (resident < 0) {
+      int per_cu = 0, dev = 0;
+      KF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, icp_level_kernel, 256, 0));
+      KF_HIP(hipGetDevice(&dev));
+      hipDeviceProp_t prop{};
+      KF_HIP(hipGetDeviceProperties(&prop, dev));
+      resident = per_cu * prop.multiProcessorCount;
+    }
+    if (nb > resident) fail(KF_ERR_DEVICE, "icp level: the grid does not fit on the device at once");
+    hipLaunchKernelGGL(icp_level_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp, ctx->partials,
+                       ctx->ticket, iterations, level);
+    KF_HIP(hipGetLastError());
+    return;
+  }
   // the level's last trial also finishes the level (and runs the learned check)
   for (int it = 0; it < iterations; ++it) {
     const int finish = it + 1 == iterations ? level : -1;
-    if (single_launch) {
-      if (check_max_rms > 0.0F) fail(KF_ERR_INVALID_ARGUMENT, "icp level: the learned check needs two-kernel trials");
-      hipLaunchKernelGGL(icp_trial_single_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
-                         ctx->partials, ctx->ticket, finish);
-      continue;
-    }
     hipLaunchKernelGGL(icp_trial_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp,
                        ctx->partials);
     hipLaunchKernelGGL(icp_trial_solve_kernel, dim3(1), dim3(256), 0, ctx->stream, p, ctx->icp, ctx->partials, nb,

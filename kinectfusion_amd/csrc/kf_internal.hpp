@@ -427,7 +427,7 @@ struct kf_context {
   unsigned long long* fused = nullptr;
   // icp scratch
   float* partials = nullptr;      // per-block partial sums (32 floats each)
-  unsigned int* ticket = nullptr; // last-workgroup counter of the single-launch trial (zero between launches)
+  unsigned int* ticket = nullptr; // one-launch ICP level: ticket (zero between trials), epoch, timeout flag
   kf::IcpState* icp = nullptr;    // device
   kf::IcpState* icp_host = nullptr;  // pinned mirror for results
   double* trajectory = nullptr;   // device, 16 doubles per frame
@@ -479,10 +479,11 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, 
 // (icp_begin_level, by one thread) -- one launch fewer per level.
 void launch_raycast(kf_context* ctx, int level, const double* pose_dev_or_null, const Rigid* pose_host,
                     bool skip_if_failed, int begin_level = -1);
-// single_launch: one kernel per trial (sweep, then the reduction and the
-// solve in the last workgroup behind device-scope fences) -- for passes that
-// rarely run and whose launches are mostly early exits (the learned warm
-// start's fallback); otherwise two kernels per trial.
+// single_launch: one kernel for the whole level (its workgroups run the
+// trials in step, with device-scope fences and a grid-wide wait between
+// trials) -- for passes that rarely run and whose launches are mostly early
+// exits (the learned warm start's fallback); otherwise two kernels per
+// trial, the last trial's second kernel also finishing the level.
 void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, int lw, int lh,
                       const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
                       int iterations, const DeviceIcpParams& p, int level_index, bool single_launch = false,
