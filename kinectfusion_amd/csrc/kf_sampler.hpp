@@ -304,12 +304,14 @@ struct Sampler {
     return true;
   }
 
-  // central_normal (trilinear.hpp:246-269).  The six samples' corner loads
-  // are issued together (two at a time for 8-byte voxels, to bound register
-  // use); the blends then run in the reference's order.
+  // central_normal (trilinear.hpp:246-269).  The corner loads of the two
+  // samples of an axis are issued together; the blends run in the
+  // reference's order.  Two samples at a time (not all six) bound the
+  // register use of the raycaster, where the normal's peak would otherwise
+  // set the register allocation, and so the occupancy, of the whole march.
   __device__ bool normal(V3 p, bool require_all, V3& out) const {
     float gr[3];
-    constexpr int kBatch = (Store::kDirect && sizeof(typename IO::Raw) == 4) ? 6 : 2;
+    constexpr int kBatch = 2;
 #pragma unroll
     for (int a0 = 0; a0 < 6; a0 += kBatch) {
       Stencil st[kBatch];
