@@ -686,14 +686,24 @@ __device__ void sym_eig6_wave(const double* eq, double* ev) {
       const double app_c = __shfl(a, pc * 7), aqq_c = __shfl(a, qc * 7), apq_c = __shfl(a, pc * 6 + qc);
       const double a_r_oc = __shfl(a, r * 6 + ocol), a_or_c = __shfl(a, orow * 6 + c),
                    a_or_oc = __shfl(a, orow * 6 + ocol);
+      // The angle (t = tan) takes the hardware reciprocal and square root:
+      // any t gives an orthogonal rotation through c = 1 / sqrt(1 + t^2),
+      // s = t c, and an angle off by a few ulp leaves a remainder of that
+      // order in A[p][q] for the next sweep.  c itself is full precision
+      // (reciprocal square root refined by three Newton steps).
       auto rotation = [](double app, double aqq, double apq, double& cc, double& ss) {
         cc = 1.0;
         ss = 0.0;
         if (apq != 0.0) {
-          const double theta = (aqq - app) / (2.0 * apq);
-          const double t = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-          cc = 1.0 / sqrt(t * t + 1.0);
-          ss = t * cc;
+          const double theta = (aqq - app) * __builtin_amdgcn_rcp(2.0 * apq);
+          const double t = (theta >= 0.0 ? 1.0 : -1.0) *
+                           __builtin_amdgcn_rcp(fabs(theta) + __builtin_amdgcn_sqrt(theta * theta + 1.0));
+          const double u = t * t + 1.0;
+          double y = __builtin_amdgcn_rsq(u);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) y = y * (1.5 - (0.5 * u) * (y * y));
+          cc = y;
+          ss = t * y;
         }
       };
       double cr, sr, cc, sc;
