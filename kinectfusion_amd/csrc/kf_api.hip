@@ -270,7 +270,8 @@ void fetch_state(kf_context* ctx) {
 
 void build_pyramid(kf_context* ctx, const uint16_t* depth) {
   launch_bilateral(ctx, depth);
-  for (int l = 0; l < ctx->cfg.levels; ++l) launch_level_maps(ctx, l);
+  if (!launch_pyramid3(ctx))
+    for (int l = 0; l < ctx->cfg.levels; ++l) launch_level_maps(ctx, l);
   ctx->levels_built = ctx->cfg.levels;
 }
 

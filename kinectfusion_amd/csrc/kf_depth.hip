@@ -78,31 +78,39 @@ __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restri
   out[static_cast<size_t>(row) * w + col] = result;
 }
 
-// BlockDownsample (depth_processing.hpp:152-187).
-__global__ void __launch_bounds__(256) downsample_kernel(const uint16_t* __restrict__ in,
-                                                          uint16_t* __restrict__ out, int w, int h,
-                                                          float scale, float max_jump) {
-  const int ow = w / 2, oh = h / 2;
-  const int col = blockIdx.x * kTx + threadIdx.x, row = blockIdx.y * kTy + threadIdx.y;
-  if (col >= ow || row >= oh) return;
+// BlockDownsample (depth_processing.hpp:152-187): the rounded mean of the
+// non-zero samples of a 2 x 2 block, or 0 when they span more than max_jump.
+__device__ inline uint16_t downsample_block(uint32_t a, uint32_t b, uint32_t c, uint32_t d, float scale,
+                                            float max_jump) {
+  const uint32_t s4[4] = {a, b, c, d};
   uint32_t sum = 0, count = 0;
   uint32_t mn = 0xFFFFU, mx = 0U;
-  for (int dy = 0; dy < 2; ++dy)
-    for (int dx = 0; dx < 2; ++dx) {
-      const uint32_t s = in[static_cast<size_t>(row * 2 + dy) * w + (col * 2 + dx)];
-      if (s == 0U) continue;
-      sum += s;
-      ++count;
-      mn = s < mn ? s : mn;
-      mx = mx < s ? s : mx;
-    }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t v = s4[i];
+    if (v == 0U) continue;
+    sum += v;
+    ++count;
+    mn = v < mn ? v : mn;
+    mx = mx < v ? v : mx;
+  }
   uint16_t r = 0;
   if (count != 0U) {
     const float hi_m = static_cast<float>(mx) / scale;
     const float lo_m = static_cast<float>(mn) / scale;
     if (!(hi_m - lo_m > max_jump)) r = static_cast<uint16_t>((sum + (count / 2U)) / count);
   }
-  out[static_cast<size_t>(row) * ow + col] = r;
+  return r;
+}
+
+__global__ void __launch_bounds__(256) downsample_kernel(const uint16_t* __restrict__ in,
+                                                          uint16_t* __restrict__ out, int w, int h,
+                                                          float scale, float max_jump) {
+  const int ow = w / 2, oh = h / 2;
+  const int col = blockIdx.x * kTx + threadIdx.x, row = blockIdx.y * kTy + threadIdx.y;
+  if (col >= ow || row >= oh) return;
+  const uint16_t* r0 = in + static_cast<size_t>(row * 2) * w + col * 2;
+  out[static_cast<size_t>(row) * ow + col] = downsample_block(r0[0], r0[1], r0[w], r0[w + 1], scale, max_jump);
 }
 
 // Vertex map (VertexProjection with the identity pose,
@@ -118,6 +126,20 @@ __device__ inline V3 vertex_of(uint16_t raw, int col, int row, const Intr& k, fl
   const Rigid id{mk3(1.0F, 0.0F, 0.0F), mk3(0.0F, 1.0F, 0.0F), mk3(0.0F, 0.0F, 1.0F),
                  mk3(0.0F, 0.0F, 0.0F)};
   return xform(id, p);
+}
+
+// NormalEstimation's interior rule: the cross product of the vertical and
+// horizontal central differences, unless a neighbour is missing or a
+// difference jumps by more than max_jump in depth.
+__device__ inline V3 normal_from(V3 c, V3 l, V3 r, V3 t, V3 b, float max_jump) {
+  V3 n = nan3();
+  if (finite3(add3(add3(add3(add3(c, l), r), t), b)) && !(fabsf(r.z - l.z) > max_jump) &&
+      !(fabsf(b.z - t.z) > max_jump)) {
+    const V3 cr = cross3(sub3(b, t), sub3(r, l));
+    const float len = norm3(cr);
+    if (len > 0.0F) n = div3(cr, len);
+  }
+  return n;
 }
 
 __global__ void __launch_bounds__(256) vertex_normal_kernel(const uint16_t* __restrict__ depth,
@@ -149,18 +171,126 @@ __global__ void __launch_bounds__(256) vertex_normal_kernel(const uint16_t* __re
   };
   const V3 c = at(0, 0);
   V3 n = nan3();
-  if (!(col == 0 || row == 0 || col + 1 >= w || row + 1 >= h)) {
-    const V3 l = at(-1, 0), r = at(1, 0), t = at(0, -1), b = at(0, 1);
-    if (finite3(add3(add3(add3(add3(c, l), r), t), b)) && !(fabsf(r.z - l.z) > max_jump) &&
-        !(fabsf(b.z - t.z) > max_jump)) {
-      const V3 cr = cross3(sub3(b, t), sub3(r, l));
-      const float len = norm3(cr);
-      if (len > 0.0F) n = div3(cr, len);
-    }
-  }
+  if (!(col == 0 || row == 0 || col + 1 >= w || row + 1 >= h))
+    n = normal_from(c, at(-1, 0), at(1, 0), at(0, -1), at(0, 1), max_jump);
   const size_t i = static_cast<size_t>(row) * w + col;
   vert[i] = st3(c);
   norm[i] = st3(n);
+}
+
+// The three-level pyramid after the bilateral pass in one launch: level-0
+// vertex and normal maps, the two 2 x 2 downsamples and their vertex and
+// normal maps.  A workgroup owns a 16 x 4 tile of level 2, so 32 x 8 of
+// level 1 and 64 x 16 of level 0; it stages level-0 depth with a 4-pixel
+// halo in LDS, derives level 1 with a 2-pixel halo and level 2 with a
+// 1-pixel halo from it (the same block rule as downsample_kernel), then
+// each level's vertices with a 1-pixel halo and the normals of its tile.
+// Every value is computed by the same arithmetic as the per-level kernels.
+constexpr int kP2x = 16, kP2y = 4;                          // level-2 tile
+constexpr int kP0w = 4 * kP2x + 8, kP0h = 4 * kP2y + 8;     // level-0 depth staged: 72 x 24
+constexpr int kP1w = 2 * kP2x + 4, kP1h = 2 * kP2y + 4;     // level-1 depth: 36 x 12
+constexpr int kP2w = kP2x + 2, kP2h = kP2y + 2;             // level-2 depth: 18 x 6
+constexpr int kV0w = 4 * kP2x + 2, kV0h = 4 * kP2y + 2;     // level-0 vertices: 66 x 18
+constexpr int kV1w = 2 * kP2x + 2, kV1h = 2 * kP2y + 2;     // level-1 vertices: 34 x 10
+
+struct PyrLevel {
+  uint16_t* depth;
+  float4* vert;
+  float4* norm;
+  int w, h;
+  Intr k;
+};
+
+__global__ void __launch_bounds__(256) pyramid3_kernel(const uint16_t* __restrict__ d0, PyrLevel l0, PyrLevel l1,
+                                                        PyrLevel l2, float scale, float lo, float hi,
+                                                        float ds_jump, float n_jump) {
+  __shared__ uint16_t s0[kP0h * kP0w];
+  __shared__ uint16_t s1[kP1h * kP1w];
+  __shared__ uint16_t s2[kP2h * kP2w];
+  __shared__ float v0[3][kV0h * kV0w];
+  __shared__ float v1[3][kV1h * kV1w];
+  __shared__ float v2[3][kP2h * kP2w];
+  const int tid = threadIdx.x;
+  const int X2 = blockIdx.x * kP2x, Y2 = blockIdx.y * kP2y;
+  const int X1 = 2 * X2, Y1 = 2 * Y2, X0 = 4 * X2, Y0 = 4 * Y2;
+  // level 0 depth, origin (X0 - 4, Y0 - 4); outside the image reads 0
+  for (int i = tid; i < kP0h * kP0w; i += 256) {
+    const int gx = X0 - 4 + i % kP0w, gy = Y0 - 4 + i / kP0w;
+    s0[i] = (gx >= 0 && gy >= 0 && gx < l0.w && gy < l0.h) ? d0[static_cast<size_t>(gy) * l0.w + gx] : 0;
+  }
+  __syncthreads();
+  // level 1, origin (X1 - 2, Y1 - 2): its 2 x 2 level-0 block starts at s0 (2 lx, 2 ly)
+  for (int i = tid; i < kP1h * kP1w; i += 256) {
+    const int lx = i % kP1w, ly = i / kP1w, gx = X1 - 2 + lx, gy = Y1 - 2 + ly;
+    uint16_t r = 0;
+    if (gx >= 0 && gy >= 0 && gx < l1.w && gy < l1.h) {
+      const uint16_t* b = s0 + (2 * ly) * kP0w + 2 * lx;
+      r = downsample_block(b[0], b[1], b[kP0w], b[kP0w + 1], scale, ds_jump);
+      if (lx >= 2 && ly >= 2 && lx < 2 + 2 * kP2x && ly < 2 + 2 * kP2y) l1.depth[static_cast<size_t>(gy) * l1.w + gx] = r;
+    }
+    s1[i] = r;
+  }
+  __syncthreads();
+  // level 2, origin (X2 - 1, Y2 - 1): its block starts at s1 (2 lx, 2 ly)
+  for (int i = tid; i < kP2h * kP2w; i += 256) {
+    const int lx = i % kP2w, ly = i / kP2w, gx = X2 - 1 + lx, gy = Y2 - 1 + ly;
+    uint16_t r = 0;
+    if (gx >= 0 && gy >= 0 && gx < l2.w && gy < l2.h) {
+      const uint16_t* b = s1 + (2 * ly) * kP1w + 2 * lx;
+      r = downsample_block(b[0], b[1], b[kP1w], b[kP1w + 1], scale, ds_jump);
+      if (lx >= 1 && ly >= 1 && lx < 1 + kP2x && ly < 1 + kP2y) l2.depth[static_cast<size_t>(gy) * l2.w + gx] = r;
+    }
+    s2[i] = r;
+  }
+  __syncthreads();
+  // vertices with a one-pixel halo (NaN outside the image)
+  auto vertex_at = [&](const uint16_t* sd, int sw, int sx, int sy, int gx, int gy, const PyrLevel& L) {
+    if (gx < 0 || gy < 0 || gx >= L.w || gy >= L.h) return nan3();
+    return vertex_of(sd[sy * sw + sx], gx, gy, L.k, scale, lo, hi);
+  };
+  for (int i = tid; i < kV0h * kV0w; i += 256) {
+    const int lx = i % kV0w, ly = i / kV0w;
+    const V3 v = vertex_at(s0, kP0w, lx + 3, ly + 3, X0 - 1 + lx, Y0 - 1 + ly, l0);
+    v0[0][i] = v.x;
+    v0[1][i] = v.y;
+    v0[2][i] = v.z;
+  }
+  for (int i = tid; i < kV1h * kV1w; i += 256) {
+    const int lx = i % kV1w, ly = i / kV1w;
+    const V3 v = vertex_at(s1, kP1w, lx + 1, ly + 1, X1 - 1 + lx, Y1 - 1 + ly, l1);
+    v1[0][i] = v.x;
+    v1[1][i] = v.y;
+    v1[2][i] = v.z;
+  }
+  for (int i = tid; i < kP2h * kP2w; i += 256) {
+    const int lx = i % kP2w, ly = i / kP2w;
+    const V3 v = vertex_at(s2, kP2w, lx, ly, X2 - 1 + lx, Y2 - 1 + ly, l2);
+    v2[0][i] = v.x;
+    v2[1][i] = v.y;
+    v2[2][i] = v.z;
+  }
+  __syncthreads();
+  // each level's tile: vertex and normal
+  auto tile = [&](float* vx, float* vy, float* vz, int vw, int tw, int th, int X, int Y, const PyrLevel& L) {
+    for (int i = tid; i < tw * th; i += 256) {
+      const int tx = i % tw, ty = i / tw, col = X + tx, row = Y + ty;
+      if (col >= L.w || row >= L.h) continue;
+      auto at = [&](int dx, int dy) {
+        const int j = (ty + 1 + dy) * vw + tx + 1 + dx;
+        return mk3(vx[j], vy[j], vz[j]);
+      };
+      const V3 c = at(0, 0);
+      V3 n = nan3();
+      if (!(col == 0 || row == 0 || col + 1 >= L.w || row + 1 >= L.h))
+        n = normal_from(c, at(-1, 0), at(1, 0), at(0, -1), at(0, 1), n_jump);
+      const size_t o = static_cast<size_t>(row) * L.w + col;
+      L.vert[o] = st3(c);
+      L.norm[o] = st3(n);
+    }
+  };
+  tile(v0[0], v0[1], v0[2], kV0w, 4 * kP2x, 4 * kP2y, X0, Y0, l0);
+  tile(v1[0], v1[1], v1[2], kV1w, 2 * kP2x, 2 * kP2y, X1, Y1, l1);
+  tile(v2[0], v2[1], v2[2], kP2w, kP2x, kP2y, X2, Y2, l2);
 }
 
 }  // namespace
@@ -179,6 +309,22 @@ void launch_bilateral(kf_context* ctx, const uint16_t* raw) {
   hipLaunchKernelGGL(bilateral_kernel, grid, block, 0, ctx->stream, raw, L.depth, L.w, L.h,
                      c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr);
   KF_HIP(hipGetLastError());
+}
+
+bool launch_pyramid3(kf_context* ctx) {
+  const kf_config& c = ctx->cfg;
+  if (c.levels != 3 || c.width % 4 != 0 || c.height % 4 != 0) return false;
+  auto level = [&](int l) {
+    LevelBuffers& L = ctx->lv[l];
+    return PyrLevel{L.depth, L.vert, L.norm, L.w, L.h, L.k};
+  };
+  const PyrLevel l2 = level(2);
+  const dim3 grid((l2.w + kP2x - 1) / kP2x, (l2.h + kP2y - 1) / kP2y);
+  hipLaunchKernelGGL(pyramid3_kernel, grid, dim3(256), 0, ctx->stream, ctx->lv[0].depth, level(0), level(1), l2,
+                     c.depth_scale, c.min_depth, c.max_depth, c.max_downsample_depth_jump,
+                     c.max_normal_depth_jump);
+  KF_HIP(hipGetLastError());
+  return true;
 }
 
 void launch_level_maps(kf_context* ctx, int level) {

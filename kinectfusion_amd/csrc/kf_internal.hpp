@@ -472,6 +472,9 @@ void hip_check(hipError_t err, const char* what);
 // Launch helpers implemented per stage.
 void launch_bilateral(kf_context* ctx, const uint16_t* raw);
 void launch_level_maps(kf_context* ctx, int level);
+// All three levels' maps after the bilateral pass in one launch (3 levels, a
+// camera size divisible by 4); false when it does not apply.
+bool launch_pyramid3(kf_context* ctx);
 // end_learned: the pixel pass also ends the learned warm start's frame.
 void launch_integrate(kf_context* ctx, const kf_depth_frame& frame, bool count, bool from_state,
                       bool end_frame = false, bool accumulate = false, bool end_learned = false);
