@@ -120,53 +120,12 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
           len += base_step;
           continue;
         }
-        if (cur_free) {  // every corner reads exactly 1
+        if (cur_free) {  // every corner reads exactly 1: no read, one free step
           if (have_prev && prev_t < 0.0F) break;
-          // This sample and the following ones while they stay in free blocks
-          // and in range: the block bits of the next kAhead - 1 samples load
-          // together, so a run through free space costs one memory round
-          // trip per kAhead samples.
-          constexpr int kAhead = 8;
-          float ls[kAhead + 1];
-          ls[0] = len;
-#pragma unroll
-          for (int i = 1; i <= kAhead; ++i) ls[i] = ls[i - 1] + free_step;
-          int fbs[kAhead];
-          fbs[0] = fb;
-#pragma unroll
-          for (int i = 1; i < kAhead; ++i) {
-            fbs[i] = -1;
-            if (!(ls[i] <= rp.max_depth) || ls[i] > len_stop) continue;
-            const V3 gi = vol.grid_coords(add3(pose.t, scale3(dir, ls[i])));
-            if (!vol.contains(gi)) continue;
-            fbs[i] = bm.flat(static_cast<int>(gi.x) >> 3, static_cast<int>(gi.y) >> 3, static_cast<int>(gi.z) >> 3);
-          }
-          bool fr[kAhead];
-          fr[0] = true;
-          // unconditional loads (a sample out of range reads word 0), so that
-          // all of them are in flight before the first is needed
-          uint32_t fw[kAhead];
-#pragma unroll
-          for (int i = 1; i < kAhead; ++i) fw[i] = free_bits[fbs[i] >= 0 ? fbs[i] >> 5 : 0];
-#pragma unroll
-          for (int i = 1; i < kAhead; ++i) fr[i] = fbs[i] >= 0 && ((fw[i] >> (fbs[i] & 31)) & 1U) != 0U;
-          float last = ls[0], next = ls[1];
-          int last_fb = fb;
-          bool run = true;
-#pragma unroll
-          for (int i = 1; i < kAhead; ++i) {
-            run = run && fr[i];
-            if (run) {
-              last = ls[i];
-              next = ls[i + 1];
-              last_fb = fbs[i];
-            }
-          }
           have_prev = true;
           prev_t = 1.0F;
-          prev_p = add3(pose.t, scale3(dir, last));
-          len = next;
-          cur_blk = last_fb;
+          prev_p = point;
+          len += free_step;
           continue;
         }
       }
