@@ -4,7 +4,7 @@ Bars: integer and index work bit-exact; float TSDF, raycast maps and
 vertex/normal maps bit-exact given identical inputs (|delta| <= 1e-5 is the
 stated floor; these kernels meet it with zero difference); bilateral filter
 within one raw unit on a small fraction of pixels (the device exponential is
-a correctly rounded FP64 exp, the reference's is glibc expf); ICP sums within
+an FP64 one within 4e-16 rounded to float, the reference's is glibc expf); ICP sums within
 1e-5 relative of the FP64 oracle; poses within 1e-4 rad / 1e-4 m.
 """
 import numpy as np
@@ -41,19 +41,22 @@ def oracle_pyramid(ko, depth, levels=3):
 
 
 # --- depth pyramid ---------------------------------------------------------------
-def test_pyramid_parity(K, ko, scene, torch_gpu):
+@pytest.mark.parametrize("levels", [3, 2, 4])
+def test_pyramid_parity(K, ko, scene, torch_gpu, levels):
+    """3 levels: the one-launch pyramid kernel; 2 and 4: the per-level
+    downsample and vertex/normal kernels."""
     depth, _, _ = scene.frame(3)
-    p = K.Pipeline(pipeline_config(K))
+    p = K.Pipeline(pipeline_config(K, levels=levels))
     src = K.PyramidSource(p)
-    assert src.build(to_dev(torch_gpu, depth)) == 3
-    ref = oracle_pyramid(ko, depth)
+    assert src.build(to_dev(torch_gpu, depth)) == levels
+    ref = oracle_pyramid(ko, depth, levels=levels)
     l0 = src.level(0)
     gpu_f = host(l0["depth"]).view(np.uint16)
     diff = np.abs(gpu_f.astype(np.int32) - ref[0]["depth"].astype(np.int32))
     print(f"bilateral: {int((diff > 0).sum())} of {diff.size} pixels differ, max {diff.max()}")
     assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
     prev = gpu_f
-    for level in range(3):
+    for level in range(levels):
         lv = src.level(level)
         d = host(lv["depth"]).view(np.uint16)
         if level > 0:
