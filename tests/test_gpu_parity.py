@@ -104,6 +104,29 @@ def test_integration_bit_exact(K, ko, scene, torch_gpu, store, rule, color):
     p.close()
 
 
+@pytest.mark.parametrize("store", ["float", "bf16"])
+def test_integration_bit_exact_partial_bricks(K, ko, scene, torch_gpu, store):
+    """A 100^3 grid: the last brick along x (16 voxels) and the last 8^3
+    blocks along each axis are partial, so the cull, the per-wave brick
+    walk and the part bits see grid edges inside a brick."""
+    res, voxel = 100, 0.02
+    p = K.Pipeline(pipeline_config(K, res=res, voxel=voxel, store=store))
+    vol = oracle_volume(ko, res=res, voxel=voxel, kind=STORE_KIND[store])
+    for i, (depth, rgba, pose) in enumerate(frames(scene, 3)):
+        normals = oracle_pyramid(ko, depth, levels=1)[0]["normals"]
+        w2c = ko.rigid_inverse(pose)
+        n_ref = ko.integrate(vol, 1, depth, INTR, w2c, ko.integration_options(1000.0), color=rgba,
+                             normals=normals)
+        n_gpu = p.integrate(to_dev(torch_gpu, depth), w2c, color=to_dev(torch_gpu, rgba),
+                            normals=to_dev(torch_gpu, f4(normals)), count=True)
+        assert tuple(n_gpu) == n_ref, (i, n_gpu, n_ref)
+    vox, col = p.host_view()
+    assert np.array_equal(vox.view(np.uint8), vol.voxels.view(np.uint8)), first_mismatch(vox, vol.voxels)
+    assert same_bits(col, vol.colors), first_mismatch(col, vol.colors)
+    assert n_ref[0] > 0
+    p.close()
+
+
 def test_integration_distance_options(K, ko, scene, torch_gpu):
     """Non-projective distance and distance-scaled truncation take the same
     per-voxel path on both sides."""
