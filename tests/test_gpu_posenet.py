@@ -180,10 +180,11 @@ def test_learned_warm_start_tracks_and_gains(K, PN, torch_gpu):
 
 def test_fallback_pass_equals_constant_velocity_tracking(K, PN, torch_gpu):
     """With an unreachable residual bar every learned proposal is rejected:
-    the fallback pass (one launch per trial) then tracks from the
-    constant-velocity seed with the full schedule, and the trajectory is the
-    constant-velocity pipeline's own, bit for bit (the same partial rows,
-    reduction order and FP64 solve)."""
+    the fallback pass (one launch per level, its workgroups running the
+    trials in step) then tracks from the constant-velocity seed with the full
+    schedule, and the trajectory is the constant-velocity pipeline's own (two
+    launches per trial), bit for bit: the same partial rows, reduction order
+    and FP64 solve."""
     net = PN.PoseNet(PN.PoseNetConfig(seed=3))
     scene = K.SyntheticScene("room", seed=0)
     frames = [scene.frame(i * 2) for i in range(6)]
