@@ -23,10 +23,18 @@ constexpr int kMaxRadius = 8;
 // exponent.  The exponential is taken in FP64 (Tang's table-driven method,
 // relative error below 4e-16, the table staged in LDS) and rounded to
 // float, which matches a correctly rounded expf.
+//
+// R > 0: the window's radius fixed at compile time and each row's taps
+// unrolled, so their exponentials overlap; the sums still add the taps in the
+// reference's row-major order, an unusable tap (outside the image or without
+// depth, both staged as -1) adding +0, which leaves them unchanged (they are
+// never -0).  R = 0: the radius is a run-time value.
+template <int R>
 __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restrict__ in,
                                                          uint16_t* __restrict__ out, int w, int h,
-                                                         float scale, float lo, float hi, int radius,
+                                                         float scale, float lo, float hi, int radius_arg,
                                                          float ss, float sr) {
+  const int radius = R > 0 ? R : radius_arg;
   __shared__ float meters[(kTy + 2 * kMaxRadius) * (kTx + 2 * kMaxRadius)];
   __shared__ float raws[(kTy + 2 * kMaxRadius) * (kTx + 2 * kMaxRadius)];
   __shared__ double exp_table[64];
@@ -55,7 +63,25 @@ __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restri
   const int cl = (threadIdx.y + radius) * tw + threadIdx.x + radius;
   const float cm = meters[cl];
   uint16_t result = 0;
-  if (cm >= 0.0F) {
+  if (R > 0 && cm >= 0.0F) {
+    float wsum_v = 0.0F, wsum = 0.0F;
+#pragma unroll 1
+    for (int dy = -R; dy <= R; ++dy) {
+      const int base = (threadIdx.y + R + dy) * tw + threadIdx.x + R;
+#pragma unroll
+      for (int dx = -R; dx <= R; ++dx) {
+        const float sm = meters[base + dx];
+        const float pd2 = static_cast<float>((dx * dx) + (dy * dy));
+        const float dd = sm - cm;
+        const float e = (pd2 * ss) + (dd * dd * sr);
+        const float wt = static_cast<float>(exp_nonpositive(static_cast<double>(e), exp_table));
+        const bool use = sm >= 0.0F;
+        wsum_v += use ? wt * raws[base + dx] : 0.0F;
+        wsum += use ? wt : 0.0F;
+      }
+    }
+    if (wsum > 0.0F) result = static_cast<uint16_t>(static_cast<long>(roundf(wsum_v / wsum)));
+  } else if (cm >= 0.0F) {
     const int xb = max(col - radius, 0), yb = max(row - radius, 0);
     const int xe = min(col + radius, w - 1), ye = min(row + radius, h - 1);
     float wsum_v = 0.0F, wsum = 0.0F;
@@ -306,7 +332,11 @@ void launch_bilateral(kf_context* ctx, const uint16_t* raw) {
   const dim3 block(kTx, kTy), grid((L.w + kTx - 1) / kTx, (L.h + kTy - 1) / kTy);
   const float ss = -0.5F / (c.bilateral_spatial_sigma * c.bilateral_spatial_sigma);
   const float sr = -0.5F / (c.bilateral_depth_sigma * c.bilateral_depth_sigma);
-  hipLaunchKernelGGL(bilateral_kernel, grid, block, 0, ctx->stream, raw, L.depth, L.w, L.h,
+  if (c.bilateral_radius == 3)
+    hipLaunchKernelGGL(bilateral_kernel<3>, grid, block, 0, ctx->stream, raw, L.depth, L.w, L.h,
+                     c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr);
+  else
+    hipLaunchKernelGGL(bilateral_kernel<0>, grid, block, 0, ctx->stream, raw, L.depth, L.w, L.h,
                      c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr);
   KF_HIP(hipGetLastError());
 }
