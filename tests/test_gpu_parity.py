@@ -71,6 +71,22 @@ def test_pyramid_parity(K, ko, scene, torch_gpu, levels):
     p.close()
 
 
+@pytest.mark.parametrize("radius", [2, 5])
+def test_bilateral_other_radii(K, ko, scene, torch_gpu, radius):
+    """Radius 3 has an unrolled kernel instance; the others take the
+    run-time-radius instance."""
+    depth, _, _ = scene.frame(3)
+    p = K.Pipeline(pipeline_config(K, radius=radius))
+    src = K.PyramidSource(p)
+    src.build(to_dev(torch_gpu, depth))
+    ref = ko.pyramid(depth, INTR, 1000.0, levels=1, radius=radius, sigma_s=4.5, sigma_r=0.03)
+    gpu_f = host(src.level(0)["depth"]).view(np.uint16)
+    diff = np.abs(gpu_f.astype(np.int32) - ref[0]["depth"].astype(np.int32))
+    print(f"bilateral r={radius}: {int((diff > 0).sum())} of {diff.size} pixels differ, max {diff.max()}")
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    p.close()
+
+
 def test_pyramid_without_filter_copies_raw(K, ko, scene, torch_gpu):
     depth, _, _ = scene.frame(1)
     p = K.Pipeline(pipeline_config(K, radius=0))
