@@ -546,6 +546,7 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->fused = dalloc<unsigned long long>(4);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     ctx->partials = dalloc<float>(512 * 32);
+    ctx->frame_w2c = dalloc<float>(16);
     ctx->ticket = dalloc<unsigned int>(3);  // ticket, epoch, timeout flag of the one-launch ICP level
     KF_HIP(hipMemsetAsync(ctx->ticket, 0, 3 * sizeof(unsigned int), ctx->stream));
     ctx->icp = dalloc<IcpState>(1);
@@ -604,6 +605,7 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->fused);
   dfree(ctx->partials);
   dfree(ctx->ticket);
+  dfree(ctx->frame_w2c);
   dfree(ctx->icp);
   dfree(ctx->trajectory);
   dfree(ctx->prev_coarse);

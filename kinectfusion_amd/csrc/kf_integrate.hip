@@ -34,26 +34,22 @@ struct FrameParams {
   int dist_scaled;
   float trunc_scale;
   int angle_rule;
-  const IcpState* st;  // when set: w2c from the running device pose; skip on failure
+  // When st is set: w2c comes from the running device pose, and the pass is
+  // skipped in a failed frame.  The pixel pass (the frame's first launch of
+  // the pass) writes that world_to_camera (12 floats) and the skip flag (1.0
+  // to skip) to frame_w2c once; the later launches read them.
+  const IcpState* st;
+  float* frame_w2c;
 };
 
-// world_to_camera of this frame, computed once per workgroup.
+// world_to_camera of this frame (false: the frame failed, skip the pass).
 __device__ inline bool block_w2c(const FrameParams& f, Rigid& out) {
-  __shared__ float xf[12];
-  __shared__ int skip;
   if (f.st == nullptr) {
     out = f.w2c;
     return true;
   }
-  if (threadIdx.x == 0 && threadIdx.y == 0) {
-    skip = f.st->frame_failed;
-    double inv[16];
-    rigid_inverse64(f.st->pose, inv);
-    for (int i = 0; i < 12; ++i) xf[i] = static_cast<float>(inv[i]);
-  }
-  __syncthreads();
-  out = rigid_from12(xf);
-  return skip == 0;
+  out = rigid_from12(f.frame_w2c);
+  return f.frame_w2c[12] == 0.0F;
 }
 
 // 1. per-pixel record {depth m (or -1), lambda, weight, truncation}.
@@ -71,7 +67,15 @@ __global__ void __launch_bounds__(256) pixel_prep_kernel(const uint16_t* __restr
   if (blockIdx.x == 0 && blockIdx.y == 0) {
     if (threadIdx.y == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0U;
     if (threadIdx.y == 0 && threadIdx.x < 2) fused[threadIdx.x] = 0ULL;
-    if (learned_st != nullptr && threadIdx.y == 0 && threadIdx.x == 0) learned_end(learned_st);
+    if (threadIdx.y == 0 && threadIdx.x == 0) {
+      if (learned_st != nullptr) learned_end(learned_st);
+      if (f.st != nullptr) {
+        double inv[16];
+        rigid_inverse64(f.st->pose, inv);
+        for (int i = 0; i < 12; ++i) f.frame_w2c[i] = static_cast<float>(inv[i]);
+        f.frame_w2c[12] = f.st->frame_failed != 0 ? 1.0F : 0.0F;
+      }
+    }
   }
   const int col = blockIdx.x * kTile + threadIdx.x, row = blockIdx.y * kTile + threadIdx.y;
   const int tid = threadIdx.y * kTile + threadIdx.x;
@@ -1159,6 +1163,7 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
   f.trunc_scale = c.truncation_distance_scale;
   f.angle_rule = c.tsdf_rule == KF_RULE_ANGLE_WEIGHTED;
   f.st = from_state ? ctx->icp : nullptr;
+  f.frame_w2c = ctx->frame_w2c;
   const int tiles_x = (f.w + kTile - 1) / kTile, tiles_y = (f.h + kTile - 1) / kTile;
   // per-pixel and per-tile scratch is sized for the configured camera
   if (f.w > c.width || f.h > c.height)

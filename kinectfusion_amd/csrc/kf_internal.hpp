@@ -427,6 +427,7 @@ struct kf_context {
   unsigned long long* fused = nullptr;
   // icp scratch
   float* partials = nullptr;      // per-block partial sums (32 floats each)
+  float* frame_w2c = nullptr;     // the frame's world_to_camera (12 floats) + skip flag, from the pixel pass
   unsigned int* ticket = nullptr; // one-launch ICP level: ticket (zero between trials), epoch, timeout flag
   kf::IcpState* icp = nullptr;    // device
   kf::IcpState* icp_host = nullptr;  // pinned mirror for results
