@@ -458,18 +458,10 @@ __device__ void finish_level_block(IcpState* st, const DeviceIcpParams& p, int l
 __global__ void __launch_bounds__(256) icp_trial_sweep_kernel(Maps m, DeviceIcpParams p, IcpState* st,
                                                                float* __restrict__ partial) {
   __shared__ float lds[4][kSums + 1];
-  __shared__ float xf[24];
   if (st->status != 0 || st->frame_failed) return;  // latched: nothing to do
-  if (threadIdx.x == 0) {
-    double inv[16];
-    rigid_inverse64(st->model_pose, inv);
-    for (int i = 0; i < 12; ++i) {
-      xf[i] = static_cast<float>(st->pose[i]);
-      xf[12 + i] = static_cast<float>(inv[i]);
-    }
-  }
-  __syncthreads();
-  const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
+  // the trial's pose, and the level's model pose inverse (kept in float by
+  // the level's preparation)
+  const Rigid cam = rigid_from_pose64(st->pose), mod = rigid_from12(st->model_inv);
   sweep_block(m, cam, mod, p.max_dist, p.min_dot, partial, lds);
 }
 
@@ -513,11 +505,9 @@ __global__ void __launch_bounds__(256) icp_level_kernel(Maps m, DeviceIcpParams 
     if (threadIdx.x == 0) {
       go = st->status == 0 ? 1 : 0;  // latched: the level is done
       if (go != 0) {
-        double inv[16];
-        rigid_inverse64(st->model_pose, inv);
         for (int i = 0; i < 12; ++i) {
           xf[i] = static_cast<float>(st->pose[i]);
-          xf[12 + i] = static_cast<float>(inv[i]);
+          xf[12 + i] = st->model_inv[i];
         }
       }
     }

@@ -263,6 +263,7 @@ struct HashView {
 struct IcpState {
   double pose[16];        // running camera_to_world
   double model_pose[16];  // render pose of the current level's model maps
+  float model_inv[12];    // model_pose^-1 (3 x 4, row-major) in float, set with model_pose
   double best_pose[16];
   double eq[29];
   double best_eq[29];
@@ -319,6 +320,9 @@ __device__ inline void icp_begin_level(IcpState* st, int level) {
     st->level_pose0[i] = st->pose[i];
     st->best_pose[i] = st->pose[i];
   }
+  double inv[16];
+  rigid_inverse64(st->model_pose, inv);
+  for (int i = 0; i < 12; ++i) st->model_inv[i] = static_cast<float>(inv[i]);
 }
 
 // The learned warm start's check after its run (one thread): keep the run
