@@ -154,6 +154,7 @@ def cpu_baseline(scene, frames: int):
     pose = np.eye(4)
     t_total = 0.0
     timed = 0
+    poses = [pose]
     for i in range(frames + 1):
         depth, rgba, _ = scene.frame(i)
         t0 = time.perf_counter()
@@ -172,6 +173,7 @@ def cpu_baseline(scene, frames: int):
                     break
                 tracked = new_pose
             pose = tracked
+            poses.append(pose)
         if ok:
             ko.integrate(vol, 1, depth, INTR, ko.rigid_inverse(pose), opts, color=rgba, normals=pyr[0]["normals"])
         dt = time.perf_counter() - t0
@@ -179,7 +181,18 @@ def cpu_baseline(scene, frames: int):
             t_total += dt
             timed += 1
         log(f"cpu oracle frame {i}: {dt:.2f} s")
-    return timed / t_total, t_total, timed
+    return timed / t_total, t_total, timed, np.stack(poses)
+
+
+def pose_gap(a: np.ndarray, b: np.ndarray):
+    """Largest rotation angle (rad) and translation distance (m) between
+    matching camera_to_world poses of two trajectories."""
+    rot, tr = 0.0, 0.0
+    for pa, pb in zip(a, b):
+        r = pa[:3, :3].T @ pb[:3, :3]
+        rot = max(rot, float(np.arccos(np.clip((np.trace(r) - 1.0) / 2.0, -1.0, 1.0))))
+        tr = max(tr, float(np.linalg.norm(pa[:3, 3] - pb[:3, 3])))
+    return rot, tr
 
 
 def cpu_model() -> str:
@@ -267,7 +280,7 @@ def main() -> int:
         log(f"timing {name} pipeline ({voxel} voxels, {init} seed): {args.warmup} warmup + {args.steps} frames")
         dt, traj = run_pipeline(K, torch, make_config(K, voxel, init), depth_dev, color_dev, args.warmup, args.steps,
                                 net if init == "learned" else None)
-        results[name] = {"seconds": dt, "fps": args.steps / dt, "ate_m": ate(traj, gt[: len(traj)])}
+        results[name] = {"seconds": dt, "fps": args.steps / dt, "ate_m": ate(traj, gt[: len(traj)]), "traj": traj}
         log(f"{name}: {args.steps / dt:.1f} frames/s, ATE {results[name]['ate_m'] * 1000:.3f} mm")
 
     head = "headline" if "headline" in results else next(iter(results))
@@ -288,11 +301,17 @@ def main() -> int:
     cpu = None
     if not args.no_cpu:
         log(f"cpu baseline: oracle on {args.cpu_frames} tracked frames (single thread)")
-        fps_cpu, secs, n = cpu_baseline(scene, args.cpu_frames)
+        fps_cpu, secs, n, oracle_poses = cpu_baseline(scene, args.cpu_frames)
         cpu = {"value": fps_cpu, "unit": "frames/s", "cores": 1, "kind": "port",
                "sample": f"frames 1-{n} of the same 512^3 workload after integrating frame 0: pyramid, 3-level "
                          f"tracking and full-volume integration on one core ({secs:.1f} s; host: {cpu_model()}, "
                          f"nproc {os.cpu_count()})"}
+        if "fp32" in results:
+            # the plain FP32 pipeline runs the oracle's configuration: its poses for the same
+            # frames, against the oracle's FP64 tracking
+            rot, tr = pose_gap(results["fp32"]["traj"][1: n + 1], oracle_poses[1:])
+            cpu["fp32_pose_vs_oracle"] = {"frames": n, "max_rot_rad": rot, "max_trans_m": tr,
+                                          "tolerance": {"rad": 1e-4, "m": 1e-4}}
 
     r = results[head]
     line = {
