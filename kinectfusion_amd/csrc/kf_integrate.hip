@@ -997,12 +997,7 @@ __global__ void fill_table_kernel(unsigned long long* table, uint32_t* stamp, si
 template <int Kind, bool Color>
 void launch_blocks_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
   using Raw = typename VoxelIO<Kind>::Raw;
-  // 24 K waves: up to that many listed bricks (about 21 K in a 512^3 room
-  // at 640x480) each get a wave of their own, and the dispatcher starts the
-  // waves as slots free up, which balances bricks of unequal cost.  Measured
-  // faster than a grid of 2048 workgroups or than one sized to the resident
-  // waves, whose waves stride over several bricks each.
-  const int grid = 6144;
+  const int grid = 2048;
   if (count) {
     hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
@@ -1037,7 +1032,8 @@ __global__ void expand_hashed_kernel(Grid g, HashView hv, const typename VoxelIO
 template <int Kind, bool Color>
 void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
   using Raw = typename VoxelIO<Kind>::Raw;
-  // 24 K waves: up to that many listed bricks (about 21 K in a 512^3 room
+  // 24 K waves (the launch sees no brick count: the list is built on the
+  // device): up to that many listed bricks (about 21 K in a 512^3 room
   // at 640x480) each get a wave of their own, and the dispatcher starts the
   // waves as slots free up, which balances bricks of unequal cost.  Measured
   // faster than a grid of 2048 workgroups or than one sized to the resident
