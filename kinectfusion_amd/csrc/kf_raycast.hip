@@ -42,8 +42,11 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
     if (state->frame_failed) return;
     pose = rigid_from_pose64(state->pose);
   }
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  // each wave of the 16x16 tile takes an 8x8 quadrant: neighbouring rays
+  // march alike, so a squarer wave diverges less than a 16x4 strip
+  const int t = static_cast<int>(threadIdx.x);
+  const int x = blockIdx.x * 16 + ((t >> 6) & 1) * 8 + (t & 7);
+  const int y = blockIdx.y * 16 + (t >> 7) * 8 + ((t >> 3) & 7);
   if (x >= rp.w || y >= rp.h) return;
   V3 P = nan3(), N = nan3();
   uint32_t C = 0U;
