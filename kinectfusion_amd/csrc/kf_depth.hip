@@ -212,12 +212,12 @@ __global__ void __launch_bounds__(256) vertex_normal_kernel(const uint16_t* __re
 // 1-pixel halo from it (the same block rule as downsample_kernel), then
 // each level's vertices with a 1-pixel halo and the normals of its tile.
 // Every value is computed by the same arithmetic as the per-level kernels.
-constexpr int kP2x = 16, kP2y = 4;                          // level-2 tile
-constexpr int kP0w = 4 * kP2x + 8, kP0h = 4 * kP2y + 8;     // level-0 depth staged: 72 x 24
-constexpr int kP1w = 2 * kP2x + 4, kP1h = 2 * kP2y + 4;     // level-1 depth: 36 x 12
-constexpr int kP2w = kP2x + 2, kP2h = kP2y + 2;             // level-2 depth: 18 x 6
-constexpr int kV0w = 4 * kP2x + 2, kV0h = 4 * kP2y + 2;     // level-0 vertices: 66 x 18
-constexpr int kV1w = 2 * kP2x + 2, kV1h = 2 * kP2y + 2;     // level-1 vertices: 34 x 10
+constexpr int kP2x = 8, kP2y = 4;                           // level-2 tile
+constexpr int kP0w = 4 * kP2x + 8, kP0h = 4 * kP2y + 8;     // level-0 depth staged: 40 x 24
+constexpr int kP1w = 2 * kP2x + 4, kP1h = 2 * kP2y + 4;     // level-1 depth: 20 x 12
+constexpr int kP2w = kP2x + 2, kP2h = kP2y + 2;             // level-2 depth: 10 x 6
+constexpr int kV0w = 4 * kP2x + 2, kV0h = 4 * kP2y + 2;     // level-0 vertices: 34 x 18
+constexpr int kV1w = 2 * kP2x + 2, kV1h = 2 * kP2y + 2;     // level-1 vertices: 18 x 10
 
 struct PyrLevel {
   uint16_t* depth;
