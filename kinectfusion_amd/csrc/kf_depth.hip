@@ -206,8 +206,8 @@ __global__ void __launch_bounds__(256) vertex_normal_kernel(const uint16_t* __re
 
 // The three-level pyramid after the bilateral pass in one launch: level-0
 // vertex and normal maps, the two 2 x 2 downsamples and their vertex and
-// normal maps.  A workgroup owns a 16 x 4 tile of level 2, so 32 x 8 of
-// level 1 and 64 x 16 of level 0; it stages level-0 depth with a 4-pixel
+// normal maps.  A workgroup owns an 8 x 4 tile of level 2, so 16 x 8 of
+// level 1 and 32 x 16 of level 0; it stages level-0 depth with a 4-pixel
 // halo in LDS, derives level 1 with a 2-pixel halo and level 2 with a
 // 1-pixel halo from it (the same block rule as downsample_kernel), then
 // each level's vertices with a 1-pixel halo and the normals of its tile.
