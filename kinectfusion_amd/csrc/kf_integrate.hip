@@ -234,7 +234,7 @@ __device__ inline bool brick_may_fuse(const Grid& g, const FrameParams& f, const
 // One thread per brick for kCullBricks bricks in a row; the workgroup's kept
 // bricks take one range of the list (one atomic per workgroup), in brick
 // order.
-constexpr int kCullBricks = 4;
+constexpr int kCullBricks = 2;
 
 __global__ void __launch_bounds__(256) brick_cull_kernel(Grid g, FrameParams f,
                                                           const float* __restrict__ tile_max,
