@@ -44,9 +44,17 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
   }
   // each wave of the 16x16 tile takes an 8x8 quadrant: neighbouring rays
   // march alike, so a squarer wave diverges less than a 16x4 strip
+  // Workgroups are dealt to the 8 XCDs round-robin by their linear id, and
+  // each XCD has its own L2: XCD k takes the k-th contiguous run of tiles
+  // (row-major), so the rays that share volume blocks share an L2.  The map
+  // is a bijection of [0, tiles) for any tile count.
+  const int tiles = static_cast<int>(gridDim.x * gridDim.y);
+  const int id = static_cast<int>(blockIdx.y * gridDim.x + blockIdx.x);
+  const int q = tiles >> 3, r = tiles & 7, k = id & 7;
+  const int tile = k * q + min(k, r) + (id >> 3);
   const int t = static_cast<int>(threadIdx.x);
-  const int x = blockIdx.x * 16 + ((t >> 6) & 1) * 8 + (t & 7);
-  const int y = blockIdx.y * 16 + (t >> 7) * 8 + ((t >> 3) & 7);
+  const int x = (tile % static_cast<int>(gridDim.x)) * 16 + ((t >> 6) & 1) * 8 + (t & 7);
+  const int y = (tile / static_cast<int>(gridDim.x)) * 16 + (t >> 7) * 8 + ((t >> 3) & 7);
   if (x >= rp.w || y >= rp.h) return;
   V3 P = nan3(), N = nan3();
   uint32_t C = 0U;
