@@ -1037,8 +1037,9 @@ void launch_integrate_t(kf_context* ctx, const FrameParams& f, const uint32_t* c
   // at 640x480) each get a wave of their own, and the dispatcher starts the
   // waves as slots free up, which balances bricks of unequal cost.  Measured
   // faster than a grid of 2048 workgroups or than one sized to the resident
-  // waves, whose waves stride over several bricks each.
-  const int grid = 6144;
+  // waves, whose waves stride over several bricks each.  The counting
+  // variant adds two counter atomics per wave, so it keeps 2048 workgroups.
+  const int grid = count ? 2048 : 6144;
   if (count) {
     hipLaunchKernelGGL((integrate_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels),
