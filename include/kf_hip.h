@@ -57,6 +57,13 @@ enum { KF_STORAGE_DENSE = 0, KF_STORAGE_HASHED = 1 };
 enum { KF_RULE_CLASSIC = 0, KF_RULE_ANGLE_WEIGHTED = 1 };
 /* IcpDampingMode (icp_correspondence.hpp:58). */
 enum { KF_DAMPING_NONE = 0, KF_DAMPING_IDENTITY = 1, KF_DAMPING_DIAGONAL = 2 };
+/* IntegrationMode (tsdf_integration.hpp:29): the full sweep carves free
+ * space everywhere; the band integrates only the 8^3 blocks that the
+ * truncation band of some pixel crosses (TruncationBandWalk at a one-voxel
+ * step, the host BandIntegrationSweep, tsdf_integration.hpp:444-497), and
+ * voxels outside the band keep their values.  A hashed volume always
+ * integrates its band (at half-block steps, block_rep.hpp:287). */
+enum { KF_INTEGRATION_FULL = 0, KF_INTEGRATION_BAND = 1 };
 /* How kf_process_frame seeds ICP. */
 enum { KF_INIT_PREVIOUS = 0, KF_INIT_CONSTANT_VELOCITY = 1, KF_INIT_LEARNED = 2 };
 
@@ -84,12 +91,19 @@ typedef struct kf_config {
   int32_t projective_distance;
   int32_t distance_scaled_truncation;
   float truncation_distance_scale;
+  int32_t integration_mode;  /* KF_INTEGRATION_*; dense storage only */
   /* RaycastOptions */
   float ray_min_depth;
   float ray_max_depth;
   float step_scale;
   int32_t require_all_corners; /* CornerPolicy::kRequireAll */
   int32_t cell_normals;
+  /* RaycastOptions::seed_from_previous (raycasting.hpp:77,122-131): a ray
+   * starts max(ray_min_depth, |p - camera| - 2 truncation) in front of the
+   * point p its pixel found in the previous raycast into the same level
+   * slot, when that point is finite and non-zero; maps start zero-filled
+   * (kf_create, kf_reset). */
+  int32_t seed_from_previous;
   /* DepthProcessingOptions */
   int32_t levels;
   float max_normal_depth_jump;
@@ -102,7 +116,10 @@ typedef struct kf_config {
   int32_t width;
   int32_t height;
   float fx, fy, cx, cy;
-  /* hashed volume (KF_STORAGE_HASHED) */
+  /* hashed volume (KF_STORAGE_HASHED).  block_capacity 0: a quarter of
+   * the grid's 8^3 blocks (PipelineConfig::sparse_block_capacity,
+   * pipeline.hpp:40); hash_buckets 0: the smallest power of two that is at
+   * least twice the capacity. */
   int64_t hash_buckets;
   int64_t block_capacity;
   /* kf_process_frame */
@@ -234,7 +251,14 @@ void kf_icp_options_defaults(kf_icp_options* opt);
 kf_status kf_create(const kf_config* cfg, kf_context** out);
 void kf_destroy(kf_context* ctx);
 const char* kf_last_error(const kf_context* ctx);
+/* Waits for the context's stream.  Also reports, once, a tracking level of
+ * the learned warm start's fallback pass whose grid-wide wait timed out
+ * (KF_ERR_DEVICE; that frame was marked failed and not integrated). */
 kf_status kf_synchronize(kf_context* ctx);
+/* Bound of that grid-wide wait in ticks of the 100 MHz device wall clock
+ * (default 2,000,000 = 20 ms, far beyond any trial).  No reference
+ * counterpart: a diagnostic that lets a test force the timeout path. */
+kf_status kf_set_sync_timeout(kf_context* ctx, int64_t ticks);
 /* Fresh volume: tsdf 1, weight 0 (volume.hpp:23,107-108). */
 kf_status kf_reset(kf_context* ctx);
 
@@ -329,6 +353,15 @@ kf_status kf_hashed_stats(kf_context* ctx, int64_t* stats);
 /* Keys (flat x-major block indices, grid.hpp:44-59) of the allocated blocks
  * in increasing order; *count receives their number. */
 kf_status kf_hashed_blocks(kf_context* ctx, uint32_t* keys, int64_t capacity, int64_t* count);
+/* The allocated blocks themselves (SparseVolumeSnapshot,
+ * block_rep.hpp:170-182, without the dense expansion): keys as
+ * kf_hashed_blocks (host, increasing), then each block's 512 voxels in that
+ * order (host, the store's voxel layout, intra-block index
+ * (z % 8 * 8 + y % 8) * 8 + x % 8, grid.hpp:61-69) and, when the volume has
+ * colour, its 512 float4 colour voxels (host, may be NULL).  At most
+ * `capacity` blocks are copied; *count receives the number allocated. */
+kf_status kf_hashed_download(kf_context* ctx, uint32_t* keys, void* voxels, float* colors, int64_t capacity,
+                             int64_t* count);
 
 /* ---- block maps used to skip samples (no counterpart in the reference; a
  * diagnostic): one bit per 8^3 voxel block, flat x-major block order.

@@ -22,6 +22,7 @@ STORAGE_DENSE, STORAGE_HASHED = 0, 1
 RULE_CLASSIC, RULE_ANGLE_WEIGHTED = 0, 1
 DAMPING_NONE, DAMPING_IDENTITY, DAMPING_DIAGONAL = 0, 1, 2
 INIT_PREVIOUS, INIT_CONSTANT_VELOCITY, INIT_LEARNED = 0, 1, 2
+INTEGRATION_FULL, INTEGRATION_BAND = 0, 1
 
 
 class KfConfig(C.Structure):
@@ -32,8 +33,9 @@ class KfConfig(C.Structure):
         ("depth_scale", C.c_float), ("observation_weight", C.c_float), ("max_weight", C.c_float),
         ("min_depth", C.c_float), ("max_depth", C.c_float), ("projective_distance", C.c_int32),
         ("distance_scaled_truncation", C.c_int32), ("truncation_distance_scale", C.c_float),
+        ("integration_mode", C.c_int32),
         ("ray_min_depth", C.c_float), ("ray_max_depth", C.c_float), ("step_scale", C.c_float),
-        ("require_all_corners", C.c_int32), ("cell_normals", C.c_int32),
+        ("require_all_corners", C.c_int32), ("cell_normals", C.c_int32), ("seed_from_previous", C.c_int32),
         ("levels", C.c_int32), ("max_normal_depth_jump", C.c_float), ("max_downsample_depth_jump", C.c_float),
         ("bilateral_filter", C.c_int32), ("bilateral_radius", C.c_int32), ("bilateral_spatial_sigma", C.c_float),
         ("bilateral_depth_sigma", C.c_float),
@@ -111,6 +113,7 @@ HIP_SYMBOLS = [
     ("kf_destroy", None, [_vp]),
     ("kf_last_error", C.c_char_p, [_vp]),
     ("kf_synchronize", _i32, [_vp]),
+    ("kf_set_sync_timeout", _i32, [_vp, _i64]),
     ("kf_reset", _i32, [_vp]),
     ("kf_pyramid_build", _i32, [_vp, _vp, C.POINTER(_i32)]),
     ("kf_pyramid_level", _i32, [_vp, _i32, C.POINTER(KfLevelView)]),
@@ -137,6 +140,7 @@ HIP_SYMBOLS = [
     ("kf_block_bitmaps", _i32, [_vp, _vp, _vp, _vp, C.POINTER(_i64)]),
     ("kf_hashed_stats", _i32, [_vp, C.POINTER(_i64)]),
     ("kf_hashed_blocks", _i32, [_vp, _vp, _i64, C.POINTER(_i64)]),
+    ("kf_hashed_download", _i32, [_vp, _vp, _vp, _vp, _i64, C.POINTER(_i64)]),
     ("kf_posenet_config_defaults", None, [C.POINTER(KfPosenetConfig)]),
     ("kf_posenet_parameter_count", _i64, []),
     ("kf_posenet_create", _i32, [C.POINTER(KfPosenetConfig), C.POINTER(_vp)]),

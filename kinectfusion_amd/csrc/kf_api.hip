@@ -127,6 +127,7 @@ __global__ void set_identity_kernel(IcpState* st) {
   st->frame_failed = 0;
   st->failed_level = -1;
   st->frames_done = 0;
+  st->level_sync[0] = 0U;
   for (int l = 0; l < kMaxLevels; ++l) {
     st->results[l] = kf_icp_result{};
     st->results[l].status = -1;
@@ -221,6 +222,8 @@ void validate(const kf_config& c) {
     require(2 * c.block_capacity <= b, "Hash buckets must be at least twice the block capacity");
   }
   require(c.tsdf_rule == KF_RULE_CLASSIC || c.tsdf_rule == KF_RULE_ANGLE_WEIGHTED, "Unknown TSDF rule");
+  require(c.integration_mode == KF_INTEGRATION_FULL || c.integration_mode == KF_INTEGRATION_BAND,
+          "Unknown integration mode");
   // TsdfIntegrator::validated (tsdf_integration.cpp) and depth options
   require(c.depth_scale > 0.0F, "Depth scale must be positive");
   require(c.min_depth >= 0.0F && c.max_depth > c.min_depth, "Depth range is invalid");
@@ -259,6 +262,34 @@ void validate(const kf_config& c) {
   }
 }
 
+// PipelineConfig::sparse_block_capacity 0 = a quarter of the grid's blocks
+// (pipeline.hpp:40); hash_buckets 0 = the smallest power of two that is at
+// least twice the capacity (and at least 2^10).
+void resolve_hashed_sizes(kf_config& c) {
+  if (c.storage != KF_STORAGE_HASHED) return;
+  if (c.block_capacity == 0) {
+    const int64_t nb = ((c.resolution[0] + 7) / 8) * ((c.resolution[1] + 7) / 8) * ((c.resolution[2] + 7) / 8);
+    c.block_capacity = std::max<int64_t>(1, nb / 4);
+  }
+  if (c.hash_buckets == 0) {
+    int64_t b = 1024;
+    while (b < 2 * c.block_capacity && b < (int64_t{1} << 30)) b *= 2;
+    c.hash_buckets = b;
+  }
+}
+
+// Raycast maps start zero-filled: a zero point is "no previous surface" for
+// RaycastOptions::seed_from_previous (raycasting.hpp:122-131).
+void clear_maps(kf_context* ctx) {
+  for (int l = 0; l < ctx->cfg.levels; ++l) {
+    const LevelBuffers& L = ctx->lv[l];
+    const size_t n = static_cast<size_t>(L.w) * L.h;
+    KF_HIP(hipMemsetAsync(L.model_pts, 0, n * sizeof(float4), ctx->stream));
+    KF_HIP(hipMemsetAsync(L.model_nrm, 0, n * sizeof(float4), ctx->stream));
+    KF_HIP(hipMemsetAsync(L.model_col, 0, n * sizeof(uint32_t), ctx->stream));
+  }
+}
+
 void upload_pose(kf_context* ctx, const double* pose) {
   KF_HIP(hipMemcpyAsync(ctx->icp->pose, pose, 16 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
 }
@@ -266,6 +297,24 @@ void upload_pose(kf_context* ctx, const double* pose) {
 void fetch_state(kf_context* ctx) {
   KF_HIP(hipMemcpyAsync(ctx->icp_host, ctx->icp, sizeof(IcpState), hipMemcpyDeviceToHost, ctx->stream));
   KF_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+// A one-launch ICP level whose grid-wide wait timed out has marked its frame
+// failed (not integrated, not reported tracked) and counted the event; the
+// first synchronising call after it reports it, once, as a device error.
+// Needs icp_host->sync_timeouts fresh from the device.
+void report_sync_timeouts(kf_context* ctx) {
+  const int n = ctx->icp_host->sync_timeouts;
+  if (n == ctx->seen_timeouts) return;
+  ctx->seen_timeouts = n;
+  fail(KF_ERR_DEVICE,
+       "ICP: a one-launch tracking level timed out waiting for its trials (is another context tracking on "
+       "this device at the same time?); the frame was not integrated");
+}
+
+void fetch_state_checked(kf_context* ctx) {
+  fetch_state(ctx);
+  report_sync_timeouts(ctx);
 }
 
 void build_pyramid(kf_context* ctx, const uint16_t* depth) {
@@ -464,7 +513,9 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
   *out = nullptr;
   kf_context* ctx = nullptr;
   const kf_status s = guard(nullptr, [&] {
-    validate(*cfg);
+    kf_config resolved = *cfg;
+    resolve_hashed_sizes(resolved);
+    validate(resolved);
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
       fail(KF_ERR_UNSUPPORTED, "no HIP device is available; this library has no host fallback");
@@ -475,7 +526,7 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       fail(KF_ERR_UNSUPPORTED, std::string("device is ") + prop.gcnArchName + "; built for gfx950 only");
     ctx = new kf_context();
-    ctx->cfg = *cfg;
+    ctx->cfg = resolved;
     ctx->device = dev;
     const kf_config& c = ctx->cfg;
     KF_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -541,14 +592,17 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
       ctx->part_bits = dalloc<uint32_t>(ctx->part_words);
       ctx->free_bits = dalloc<uint32_t>(ctx->block_words);
       KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
+      if (c.storage == KF_STORAGE_DENSE && c.integration_mode == KF_INTEGRATION_BAND) {
+        ctx->band_bits = dalloc<uint32_t>(ctx->block_words);
+        KF_HIP(hipMemsetAsync(ctx->band_bits, 0, ctx->block_words * 4, ctx->stream));
+        ctx->band_list = dalloc<uint32_t>(static_cast<size_t>(bm.bx) * bm.by * bm.bz);
+      }
     }
     ctx->counters = dalloc<uint32_t>(4);
     ctx->fused = dalloc<unsigned long long>(4);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
     ctx->partials = dalloc<float>(512 * 32);
     ctx->frame_w2c = dalloc<float>(16);
-    ctx->ticket = dalloc<unsigned int>(3);  // ticket, epoch, timeout flag of the one-launch ICP level
-    KF_HIP(hipMemsetAsync(ctx->ticket, 0, 3 * sizeof(unsigned int), ctx->stream));
     ctx->icp = dalloc<IcpState>(1);
     KF_HIP(hipMemsetAsync(ctx->icp, 0, sizeof(IcpState), ctx->stream));
     KF_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->icp_host), sizeof(IcpState), hipHostMallocDefault));
@@ -561,6 +615,7 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
     ctx->traj_capacity = 1 << 15;  // frames before the first regrowth
     ctx->trajectory = dalloc<double>(ctx->traj_capacity * 16);
     for (auto& e : ctx->ev) KF_HIP(hipEventCreate(&e));
+    clear_maps(ctx);
     launch_fill_voxels(ctx);
     if (hashed(ctx)) launch_fill_pool(ctx);
     launch_skip_map(ctx);
@@ -597,6 +652,8 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->pix);
   dfree(ctx->tile_max);
   dfree(ctx->bricks);
+  dfree(ctx->band_bits);
+  dfree(ctx->band_list);
   dfree(ctx->obs_bits);
   dfree(ctx->skip_bits);
   dfree(ctx->part_bits);
@@ -604,7 +661,6 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->counters);
   dfree(ctx->fused);
   dfree(ctx->partials);
-  dfree(ctx->ticket);
   dfree(ctx->frame_w2c);
   dfree(ctx->icp);
   dfree(ctx->trajectory);
@@ -627,7 +683,21 @@ const char* kf_last_error(const kf_context* ctx) {
 }
 
 kf_status kf_synchronize(kf_context* ctx) {
-  return guard(ctx, [&] { KF_HIP(hipStreamSynchronize(ctx->stream)); });
+  return guard(ctx, [&] {
+    KF_HIP(hipMemcpyAsync(&ctx->icp_host->sync_timeouts, &ctx->icp->sync_timeouts, sizeof(int),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    report_sync_timeouts(ctx);
+  });
+}
+
+kf_status kf_set_sync_timeout(kf_context* ctx, int64_t ticks) {
+  return guard(ctx, [&] {
+    require(ticks >= 0, "kf_set_sync_timeout: ticks must be >= 0");
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->sync_timeout_ticks = ticks;
+    drop_graph(ctx);  // the captured frame holds the bound as a kernel argument
+  });
 }
 
 kf_status kf_reset(kf_context* ctx) {
@@ -638,6 +708,7 @@ kf_status kf_reset(kf_context* ctx) {
     KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
     launch_skip_map(ctx);
     KF_HIP(hipMemsetAsync(ctx->fused, 0, 4 * sizeof(unsigned long long), ctx->stream));
+    clear_maps(ctx);
     hipLaunchKernelGGL(set_identity_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp);
     KF_HIP(hipGetLastError());
     ctx->frames = 0;
@@ -713,7 +784,7 @@ kf_status kf_icp_reduce(kf_context* ctx, const float* lv, const float* ln, int32
                            lw, lh, reinterpret_cast<const float4*>(mv),
                            reinterpret_cast<const float4*>(mn), mw, mh, Intr{k[0], k[1], k[2], k[3]},
                            camera, model, max_dist, min_dot);
-    fetch_state(ctx);
+    fetch_state_checked(ctx);
     for (int i = 0; i < 29; ++i) sums[i] = static_cast<double>(ctx->icp_host->sums32[i]);
     *count = static_cast<int64_t>(__builtin_bit_cast(uint32_t, ctx->icp_host->sums32[29]));
   });
@@ -749,6 +820,7 @@ kf_status kf_icp_track_maps(kf_context* ctx, const float* lv, const float* ln, i
                           hipMemcpyDeviceToDevice, ctx->stream));
     ctx->icp_host->frame_failed = 0;
     KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
+    report_sync_timeouts(ctx);
   });
 }
 
@@ -766,6 +838,7 @@ kf_status kf_track(kf_context* ctx, const kf_icp_options* opt, double* pose_inou
     KF_HIP(hipMemcpyAsync(ctx->icp->pose, ctx->icp->prev_pose, 16 * sizeof(double),
                           hipMemcpyDeviceToDevice, ctx->stream));
     KF_HIP(hipMemsetAsync(&ctx->icp->frame_failed, 0, sizeof(int), ctx->stream));
+    report_sync_timeouts(ctx);
   });
 }
 
@@ -829,6 +902,7 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
         result->proposal_rotation_error = ctx->icp_host->proposal_err_r;
         result->learned_rms = ctx->icp_host->learned_rms;
       }
+      report_sync_timeouts(ctx);
     }
   });
 }
@@ -1011,6 +1085,41 @@ extern "C" kf_status kf_hashed_blocks(kf_context* ctx, uint32_t* keys, int64_t c
   });
 }
 
+extern "C" kf_status kf_hashed_download(kf_context* ctx, uint32_t* keys, void* voxels, float* colors,
+                                        int64_t capacity, int64_t* count) {
+  return guard(ctx, [&] {
+    require(hashed(ctx), "kf_hashed_download: the volume is dense");
+    require(capacity >= 0, "kf_hashed_download: negative capacity");
+    int64_t stats[4];
+    const kf_status st = kf_hashed_stats(ctx, stats);
+    if (st != KF_OK) fail(st, ctx->error);
+    const int64_t n = stats[0];
+    if (count != nullptr) *count = n;
+    const int64_t m = std::min(n, capacity);
+    if (m == 0) return;
+    // slots 0..n-1 hold the allocated blocks; stage them, then reorder by key
+    std::vector<uint32_t> key_of(static_cast<size_t>(n));
+    KF_HIP(hipMemcpyAsync(key_of.data(), ctx->block_of_slot, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    KF_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<int64_t> order(static_cast<size_t>(n));
+    for (int64_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return key_of[a] < key_of[b]; });
+    const size_t vb = kBlockVoxels * voxel_bytes(ctx->cfg.voxel_store), cb = kBlockVoxels * sizeof(float4);
+    auto gather = [&](const void* dev, size_t per_block, void* out) {
+      std::vector<uint8_t> stage(static_cast<size_t>(n) * per_block);
+      KF_HIP(hipMemcpyAsync(stage.data(), dev, stage.size(), hipMemcpyDeviceToHost, ctx->stream));
+      KF_HIP(hipStreamSynchronize(ctx->stream));
+      for (int64_t j = 0; j < m; ++j)
+        std::memcpy(static_cast<uint8_t*>(out) + j * per_block, stage.data() + order[j] * per_block, per_block);
+    };
+    if (keys != nullptr)
+      for (int64_t j = 0; j < m; ++j) keys[j] = key_of[order[j]];
+    if (voxels != nullptr) gather(ctx->voxels, vb, voxels);
+    if (colors != nullptr && ctx->colors != nullptr) gather(ctx->colors, cb, colors);
+  });
+}
+
 extern "C" kf_status kf_block_bitmaps(kf_context* ctx, uint32_t* observed, uint32_t* skip, uint32_t* free_space,
                                       int64_t* words) {
   return guard(ctx, [&] {
@@ -1033,6 +1142,10 @@ extern "C" kf_status kf_attach_posenet(kf_context* ctx, kf_posenet* net) {
       const LevelBuffers& L = ctx->lv[ctx->cfg.levels - 1];
       require(ctx->cfg.levels >= 3 && posenet_input_matches(L.w, L.h),
               "kf_attach_posenet: the coarsest pyramid level must be 160 x 120");
+      // the proposal reads the coarse depth in the context's units, and the
+      // frame graph reads the weights that training on the network's
+      // stream writes: no training step may still be in flight
+      posenet_prepare_attach(net, ctx->cfg.depth_scale);
     }
     KF_HIP(hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);

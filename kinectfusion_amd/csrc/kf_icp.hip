@@ -486,20 +486,22 @@ __device__ inline bool last_block(unsigned int* ticket) {
 // epoch word with a device-scope release; the others wait for the epoch
 // with a device-scope acquire.  The grid is at most kMaxBlocks workgroups,
 // all resident at once (checked on the host).  A wait that outlasts
-// kSyncTimeoutTicks of the 100 MHz wall clock (far beyond any trial) gives
-// up: the workgroup raises sync[2] and exits, so every wave ends.
-// sync: [0] ticket, [1] epoch (only ever advanced: a launch reads its base
-// before any workgroup can advance it), [2] timeout flag.
-constexpr long long kSyncTimeoutTicks = 2000000;  // 20 ms
-
+// timeout_ticks of the 100 MHz wall clock (far beyond any trial by default)
+// gives up: the workgroup marks the frame failed (it is then neither
+// integrated nor reported tracked), counts the event in st->sync_timeouts
+// for the host to report, and exits, so every wave ends.  The ticket a
+// timed-out launch leaves short is reset when the next level begins.
+// st->level_sync: [0] ticket, [1] epoch (only ever advanced: a launch reads
+// its base before any workgroup can advance it).
 __global__ void __launch_bounds__(256) icp_level_kernel(Maps m, DeviceIcpParams p, IcpState* st,
-                                                        float* __restrict__ partial, unsigned int* sync,
+                                                        float* __restrict__ partial, long long timeout_ticks,
                                                         int iterations, int level) {
   __shared__ float lds[4][kSums + 1];
   __shared__ float combined[kSums + 1];
   __shared__ float xf[24];
   __shared__ int go, timed_out;
   if (st->frame_failed) return;
+  unsigned int* sync = st->level_sync;
   const unsigned int base = __hip_atomic_load(&sync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
   for (int it = 0; it < iterations; ++it) {
     if (threadIdx.x == 0) {
@@ -533,9 +535,11 @@ __global__ void __launch_bounds__(256) icp_level_kernel(Maps m, DeviceIcpParams 
         const long long t0 = wall_clock64();
         while (static_cast<int>(__hip_atomic_load(&sync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) <
                0) {
-          if (wall_clock64() - t0 > kSyncTimeoutTicks) {
+          if (wall_clock64() - t0 > timeout_ticks) {
             timed_out = 1;
-            atomicOr(&sync[2], 1U);
+            st->frame_failed = 1;
+            st->failed_level = level;
+            atomicAdd(&st->sync_timeouts, 1);
             break;
           }
           __builtin_amdgcn_s_sleep(2);
@@ -825,7 +829,7 @@ void launch_icp_level(kf_context* ctx, const float4* lv_v, const float4* lv_n, i
     }
     if (nb > resident) fail(KF_ERR_DEVICE, "icp level: the grid does not fit on the device at once");
     hipLaunchKernelGGL(icp_level_kernel, dim3(nb), dim3(256), 0, ctx->stream, m, p, ctx->icp, ctx->partials,
-                       ctx->ticket, iterations, level);
+                       ctx->sync_timeout_ticks, iterations, level);
     KF_HIP(hipGetLastError());
     return;
   }

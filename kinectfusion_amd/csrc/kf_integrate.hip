@@ -915,15 +915,18 @@ __global__ void __launch_bounds__(256) assign_slots_kernel(HashView hv, const ui
 }
 
 // Every voxel of the listed blocks (BlockVoxels, clipped to the grid): one
-// workgroup per block, x fastest, so pool accesses are contiguous.
-template <int Kind, bool Color, bool Count>
+// workgroup per block, x fastest, so voxel accesses are contiguous.  Hashed
+// (Dense = false): the list holds pool slots, block_of_slot their keys, and
+// a block's voxels are its pool slot's 512.  Dense band: the list holds
+// block keys and voxels sit at their dense indices.  n: the list's length.
+template <int Kind, bool Color, bool Count, bool Dense>
 __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FrameParams f, const float4* __restrict__ pix,
                                                                 const uint32_t* __restrict__ color_img,
                                                                 typename VoxelIO<Kind>::Raw* __restrict__ pool,
                                                                 float4* __restrict__ cpool,
                                                                 const uint32_t* __restrict__ block_of_slot,
                                                                 const uint32_t* __restrict__ work,
-                                                                const uint32_t* __restrict__ sp,
+                                                                const uint32_t* __restrict__ count,
                                                                 unsigned long long* __restrict__ fused,
                                                                 uint32_t* __restrict__ obs_bits, BlockMap bm,
                                                                 uint32_t* __restrict__ part_bits) {
@@ -931,11 +934,11 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
   __shared__ int all_ok, all_complete;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
-  const uint32_t n = sp[1];
+  const uint32_t n = *count;
   unsigned int my_fused = 0, my_color = 0;
   for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
     const uint32_t slot = work[li];
-    const uint32_t key = block_of_slot[slot];
+    const uint32_t key = Dense ? slot : block_of_slot[slot];
     const int bx = static_cast<int>(key % static_cast<uint32_t>(bm.bx));
     const int by = static_cast<int>((key / static_cast<uint32_t>(bm.bx)) % static_cast<uint32_t>(bm.by));
     const int bz = static_cast<int>(key / static_cast<uint32_t>(bm.bx * bm.by));
@@ -955,7 +958,8 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
       ys[pass] = by * 8 + ((i >> 3) & 7);
       zs[pass] = bz * 8 + (i >> 6);
       inside[pass] = xs[pass] < g.nx && ys[pass] < g.ny && zs[pass] < g.nz;
-      vis[pass] = static_cast<size_t>(slot) * kBlockVoxels + i;
+      if constexpr (Dense) vis[pass] = inside[pass] ? g.flat(xs[pass], ys[pass], zs[pass]) : 0;
+      else vis[pass] = static_cast<size_t>(slot) * kBlockVoxels + i;
     }
     bool any = false, ones_ok = true, complete = true;
     fuse_batch<Kind, Color, Count, 2>(g, f, w2c, pix, color_img, pool, cpool, xs, ys, zs, vis, inside, my_fused,
@@ -994,20 +998,93 @@ __global__ void fill_table_kernel(unsigned long long* table, uint32_t* stamp, si
   if (blockIdx.x == 0 && threadIdx.x < 8) sp[threadIdx.x] = 0U;
 }
 
-template <int Kind, bool Color>
-void launch_blocks_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
+template <int Kind, bool Color, bool Dense>
+void launch_blocks_kernel(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count,
+                          const uint32_t* work, const uint32_t* n) {
   using Raw = typename VoxelIO<Kind>::Raw;
   const int grid = 2048;
   if (count) {
-    hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, true>), dim3(grid), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, true, Dense>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
-                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid),
+                       ctx->block_of_slot, work, n, ctx->fused, ctx->obs_bits, block_map(ctx->grid),
                        ctx->part_bits);
   } else {
-    hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, false>), dim3(grid), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((integrate_blocks_kernel<Kind, Color, false, Dense>), dim3(grid), dim3(256), 0, ctx->stream,
                        ctx->grid, f, ctx->pix, color_img, static_cast<Raw*>(ctx->voxels), ctx->colors,
-                       ctx->block_of_slot, ctx->work, ctx->sparse, ctx->fused, ctx->obs_bits, block_map(ctx->grid),
+                       ctx->block_of_slot, work, n, ctx->fused, ctx->obs_bits, block_map(ctx->grid),
                        ctx->part_bits);
+  }
+}
+
+template <int Kind, bool Color>
+void launch_blocks_t(kf_context* ctx, const FrameParams& f, const uint32_t* color_img, bool count) {
+  if (hashed(ctx)) launch_blocks_kernel<Kind, Color, false>(ctx, f, color_img, count, ctx->work, ctx->sparse + 1);
+  else launch_blocks_kernel<Kind, Color, true>(ctx, f, color_img, count, ctx->band_list, ctx->counters);
+}
+
+// ---- dense band integration: BandIntegrationSweep<kHost>
+// (tsdf_integration.hpp:444-497) -------------------------------------------
+// Marks the blocks of every pixel's truncation band in a bitmap
+// (TruncationBandWalk::visit_pixel at a one-voxel step, the host sweep's
+// step, tsdf_integration.hpp:462-463), skipping a block equal to the pixel's
+// previous visit.
+__global__ void __launch_bounds__(256) band_mark_kernel(Grid g, FrameParams f, const uint16_t* __restrict__ depth,
+                                                         uint32_t* __restrict__ bits, BlockMap bm, float step) {
+  Rigid w2c;
+  if (!block_w2c(f, w2c)) return;
+  const Rigid c2w = rigid_inverse_f(w2c);
+  const int col = blockIdx.x * kTile + threadIdx.x, row = blockIdx.y * kTile + threadIdx.y;
+  if (col >= f.w || row >= f.h) return;
+  float measured;
+  if (!depth_in_range(depth[static_cast<size_t>(row) * f.w + col], f.scale, f.lo, f.hi, measured)) return;
+  const float band = f.dist_scaled ? g.trunc + (f.trunc_scale * measured) : g.trunc;
+  const int samples = static_cast<int>((2.0F * band) / step) + 1;
+  const float inv_voxel = 1.0F / g.voxel;
+  int previous = -1;
+  for (int s = 0; s <= samples; ++s) {
+    const float d = (measured - band) + (static_cast<float>(s) * step);
+    const V3 world = xform(c2w, back_project(f.k, static_cast<float>(col), static_cast<float>(row), d));
+    const V3 gc = scale3(sub3(world, mk3(g.ox, g.oy, g.oz)), inv_voxel);
+    if (!(gc.x >= 0.0F && gc.y >= 0.0F && gc.z >= 0.0F && gc.x < static_cast<float>(g.nx) &&
+          gc.y < static_cast<float>(g.ny) && gc.z < static_cast<float>(g.nz)))
+      continue;
+    const int b = bm.flat(static_cast<int>(gc.x) >> 3, static_cast<int>(gc.y) >> 3, static_cast<int>(gc.z) >> 3);
+    if (b == previous) continue;
+    previous = b;
+    if (!bm.bit(bits, b)) atomicOr(&bits[b >> 5], 1U << (b & 31));
+  }
+}
+
+// The marked blocks as a work list (each once), one thread per bitmap word;
+// the word is cleared as it is read, so the bitmap starts the next frame
+// empty.  A wave's appends take one range of the list (an inclusive scan of
+// the per-lane counts).  The list order does not matter: blocks are
+// disjoint.
+__global__ void __launch_bounds__(256) band_compact_kernel(uint32_t* __restrict__ bits, int words,
+                                                            uint32_t* __restrict__ list, uint32_t* __restrict__ count) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 256 + static_cast<int>(threadIdx.x);
+  uint32_t w = 0U;
+  if (i < words) {
+    w = bits[i];
+    if (w != 0U) bits[i] = 0U;
+  }
+  const int c = __popc(w);
+  int incl = c;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  const int total = __shfl(incl, 63);
+  uint32_t base = 0U;
+  if (lane == 63 && total > 0) base = atomicAdd(count, static_cast<uint32_t>(total));
+  base = __shfl(base, 63);
+  uint32_t at = base + static_cast<uint32_t>(incl - c);
+  while (w != 0U) {
+    const int b = __ffs(static_cast<int>(w)) - 1;
+    w &= w - 1U;
+    list[at++] = static_cast<uint32_t>(i) * 32U + static_cast<uint32_t>(b);
   }
 }
 
@@ -1192,6 +1269,37 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
     hipLaunchKernelGGL(assign_slots_kernel, dim3(1024), dim3(256), 0, ctx->stream, hash_view(ctx), ctx->hash_stamp,
                        ctx->sparse, ctx->block_of_slot, ctx->work, static_cast<uint32_t>(ctx->capacity),
                        static_cast<uint32_t>(ctx->buckets));
+    if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
+    switch (c.voxel_store) {
+      case KF_VOXEL_QUANTIZED:
+        color ? launch_blocks_t<KF_VOXEL_QUANTIZED, true>(ctx, f, cimg, count)
+              : launch_blocks_t<KF_VOXEL_QUANTIZED, false>(ctx, f, cimg, count);
+        break;
+      case KF_VOXEL_BF16:
+        color ? launch_blocks_t<KF_VOXEL_BF16, true>(ctx, f, cimg, count)
+              : launch_blocks_t<KF_VOXEL_BF16, false>(ctx, f, cimg, count);
+        break;
+      default:
+        color ? launch_blocks_t<KF_VOXEL_FLOAT, true>(ctx, f, cimg, count)
+              : launch_blocks_t<KF_VOXEL_FLOAT, false>(ctx, f, cimg, count);
+        break;
+    }
+    if (timing) KF_HIP(hipEventRecord(ctx->ev[13], ctx->stream));
+    launch_skip_map(ctx, end_frame);
+    if (count && accumulate)
+      hipLaunchKernelGGL(accumulate_counts_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->fused);
+    KF_HIP(hipGetLastError());
+    return;
+  }
+  if (c.integration_mode == KF_INTEGRATION_BAND) {
+    // band walk -> block list -> listed blocks; the list count is the brick
+    // counter, which the pixel pass zeroed
+    const BlockMap bm = block_map(ctx->grid);
+    const int words = static_cast<int>(ctx->block_words);
+    hipLaunchKernelGGL(band_mark_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream, ctx->grid, f,
+                       fr.depth, ctx->band_bits, bm, ctx->grid.voxel);
+    hipLaunchKernelGGL(band_compact_kernel, dim3((words + 255) / 256), dim3(256), 0, ctx->stream, ctx->band_bits,
+                       words, ctx->band_list, ctx->counters);
     if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
     switch (c.voxel_store) {
       case KF_VOXEL_QUANTIZED:

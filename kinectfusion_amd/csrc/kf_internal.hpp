@@ -276,7 +276,9 @@ struct IcpState {
   int trials, rejected;
   int frame_failed;
   int failed_level;
-  int pad;
+  // one-launch levels whose grid-wide wait timed out, since kf_create (never
+  // cleared; the host reports each new one once)
+  int sync_timeouts;
   float sums32[30];  // last compensated FP32 reduction (29 sums + count)
   double level_pose0[16];  // pose at the start of the current level
   kf_icp_result results[kMaxLevels];
@@ -290,6 +292,10 @@ struct IcpState {
   double proposal_err_t, proposal_err_r, learned_rms;
   int used_learned;
   int pad2;
+  // grid-wide protocol of the one-launch level: [0] ticket (zero between
+  // trials, and reset when a level begins, after a timed-out launch left it
+  // short), [1] epoch (only ever advanced)
+  unsigned int level_sync[2];
 };
 
 // Prepare a tracking level (one thread): the level's result reads "not run",
@@ -297,6 +303,7 @@ struct IcpState {
 // failed frame nothing runs; a frame whose learned run was kept (marked
 // failed so that the fallback pass exits at once) keeps its results.
 __device__ inline void icp_begin_level(IcpState* st, int level) {
+  st->level_sync[0] = 0U;
   if (st->frame_failed != 0) {
     if (st->used_learned == 0) {
       st->results[level] = kf_icp_result{};
@@ -413,6 +420,10 @@ struct kf_context {
   float4* pix = nullptr;          // per pixel: depth m, lambda, weight, valid
   float* tile_max = nullptr;      // per 16x16 tile max usable depth
   uint32_t* bricks = nullptr;     // active brick list
+  // dense band integration: blocks the band marked (cleared as they are
+  // listed) and their list
+  uint32_t* band_bits = nullptr;
+  uint32_t* band_list = nullptr;
   // one bit per 8^3 voxel block: any voxel observed / block and its 26
   // neighbours all unobserved (the raycaster skips those)
   uint32_t* obs_bits = nullptr;
@@ -432,7 +443,10 @@ struct kf_context {
   // icp scratch
   float* partials = nullptr;      // per-block partial sums (32 floats each)
   float* frame_w2c = nullptr;     // the frame's world_to_camera (12 floats) + skip flag, from the pixel pass
-  unsigned int* ticket = nullptr; // one-launch ICP level: ticket (zero between trials), epoch, timeout flag
+  // bound of the one-launch ICP level's grid-wide wait, in ticks of the
+  // 100 MHz wall clock; and the sync_timeouts count already reported
+  long long sync_timeout_ticks = 2000000;  // 20 ms
+  int seen_timeouts = 0;
   kf::IcpState* icp = nullptr;    // device
   kf::IcpState* icp_host = nullptr;  // pinned mirror for results
   double* trajectory = nullptr;   // device, 16 doubles per frame
@@ -527,5 +541,8 @@ DeviceIcpParams to_device_params(const kf_icp_options& o);
 // Pose network (kf_posenet.hip): proposal for one depth pair on stream s.
 void posenet_propose(kf_posenet* net, hipStream_t s, const uint16_t* prev, const uint16_t* cur, float* twist);
 bool posenet_input_matches(int width, int height);
+// Before a context uses the network: its depth scale must be the context's,
+// and no call on the network's stream may still run.
+void posenet_prepare_attach(kf_posenet* net, float depth_scale);
 
 }  // namespace kf

@@ -824,6 +824,12 @@ void posenet_propose(kf_posenet* net, hipStream_t s, const uint16_t* prev, const
 
 bool posenet_input_matches(int width, int height) { return width == kInW && height == kInH; }
 
+void posenet_prepare_attach(kf_posenet* net, float depth_scale) {
+  if (net->cfg.depth_scale != depth_scale)
+    fail(KF_ERR_INVALID_ARGUMENT, "kf_attach_posenet: the network's depth scale differs from the context's");
+  KF_HIP(hipStreamSynchronize(net->stream));
+}
+
 }  // namespace kf
 
 extern "C" {

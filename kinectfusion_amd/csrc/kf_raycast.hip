@@ -19,6 +19,7 @@ struct RayParams {
   float min_depth, max_depth, step_scale;
   int require_all;
   int cell_normals;
+  int seed_previous;
 };
 
 __device__ inline uint32_t pixel_from_color(V3 c) {
@@ -56,6 +57,18 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
   const int x = (tile % static_cast<int>(gridDim.x)) * 16 + ((t >> 6) & 1) * 8 + (t & 7);
   const int y = (tile / static_cast<int>(gridDim.x)) * 16 + (t >> 7) * 8 + ((t >> 3) & 7);
   if (x >= rp.w || y >= rp.h) return;
+  const size_t i = static_cast<size_t>(y) * rp.w + x;
+  // RaycastOptions::seed_from_previous (raycasting.hpp:122-131): start just
+  // in front of the surface this pixel found in the previous raycast into
+  // this map (a zero point is the maps' initial fill, not a surface)
+  float seed = rp.min_depth;
+  if (rp.seed_previous != 0) {
+    const V3 previous = ld3(pts[i]);
+    if (finite3(previous) && dot3(previous, previous) > 0.0F) {
+      const float previous_depth = norm3(sub3(previous, pose.t));
+      seed = fmax_ref(rp.min_depth, previous_depth - (2.0F * vol.g.trunc));
+    }
+  }
   V3 P = nan3(), N = nan3();
   uint32_t C = 0U;
   const V3 translated = back_project(rp.k, static_cast<float>(x), static_cast<float>(y), 1.0F);
@@ -67,7 +80,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
     bool have_prev = false, hit = false;
     float prev_t = 0.0F;
     V3 prev_p = mk3(0.0F, 0.0F, 0.0F), surface = prev_p;
-    float len = rp.min_depth;
+    float len = seed;
     // Samples whose outcome is known are stepped over without touching the
     // volume, reproducing the reference's float sequence of ray lengths:
     // before the ray enters the volume box, after it has left it (it cannot
@@ -179,7 +192,6 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
       }
     }
   }
-  const size_t i = static_cast<size_t>(y) * rp.w + x;
   pts[i] = st3(P);
   nrm[i] = st3(N);
   col[i] = C;
@@ -221,6 +233,7 @@ void launch_raycast(kf_context* ctx, int level, const double* /*unused*/, const 
   rp.step_scale = c.step_scale;
   rp.require_all = c.require_all_corners;
   rp.cell_normals = c.cell_normals;
+  rp.seed_previous = c.seed_from_previous;
   const Rigid pose = pose_host != nullptr ? *pose_host : Rigid{};
   IcpState* state = from_state ? ctx->icp : nullptr;
   switch (c.voxel_store) {

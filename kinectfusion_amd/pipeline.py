@@ -71,6 +71,7 @@ class TsdfIntegrationOptions:
     projective_distance: bool = True
     distance_scaled_truncation: bool = False
     truncation_distance_scale: float = 0.01
+    mode: str = "full"                         # IntegrationMode (:29): full | band (dense storage)
 
 
 @dataclass
@@ -81,6 +82,7 @@ class RaycastOptions:
     step_scale: float = 1.0
     require_all_corners: bool = False
     cell_gradient_normals: bool = False
+    seed_from_previous: bool = False
 
 
 @dataclass
@@ -118,7 +120,7 @@ class PipelineConfig:
     count_fused: bool = False
     # hashed storage (BlockRep, block_rep.hpp:183-307)
     hash_buckets: int = 1 << 20
-    block_capacity: int = 500_000
+    block_capacity: int = 500_000              # 0: a quarter of the grid's blocks (pipeline.hpp:40)
     # learned warm start: iterations per level from the network's proposal
     # (0 skips a level) and the final RMS (m) above which the frame falls
     # back to the constant-velocity seed with icp_iterations
@@ -144,9 +146,11 @@ class PipelineConfig:
         c.projective_distance = int(i.projective_distance)
         c.distance_scaled_truncation = int(i.distance_scaled_truncation)
         c.truncation_distance_scale = i.truncation_distance_scale
+        c.integration_mode = {"full": N.INTEGRATION_FULL, "band": N.INTEGRATION_BAND}[i.mode]
         r = self.raycast
         c.ray_min_depth, c.ray_max_depth, c.step_scale = r.min_depth, r.max_depth, r.step_scale
         c.require_all_corners, c.cell_normals = int(r.require_all_corners), int(r.cell_gradient_normals)
+        c.seed_from_previous = int(r.seed_from_previous)
         d = self.depth
         c.levels = d.levels
         c.max_normal_depth_jump, c.max_downsample_depth_jump = d.max_normal_depth_jump, d.max_downsample_depth_jump
@@ -221,6 +225,30 @@ def _outcome(pose: np.ndarray, r: N.KfIcpResult) -> IcpOutcome:
     return IcpOutcome(pose, name, r)
 
 
+def _check_image(t, name: str, dtypes, shape, optional: bool = False):
+    """A device image the library reads through a raw pointer: a contiguous
+    cuda tensor of one of `dtypes` and exactly `shape` (the reference's
+    std::invalid_argument for an invalid frame, validation.hpp:9-13)."""
+    if t is None:
+        if optional:
+            return None
+        raise ValueError(f"{name}: missing")
+    if not getattr(t, "is_cuda", False):
+        raise ValueError(f"{name}: must be a device (cuda) tensor; the hip space reads device memory only")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if str(t.dtype).replace("torch.", "") not in dtypes:
+        raise ValueError(f"{name}: dtype {t.dtype} is not one of {sorted(dtypes)}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)} differs from {tuple(shape)}")
+    return t
+
+
+_DEPTH = ("int16", "uint16")
+_COLOR = ("int32", "uint32")
+_FLOAT = ("float32",)
+
+
 @dataclass
 class SurfaceMaps:
     """volume.hpp:208-219 on the device: points/normals (H, W, 4) float32,
@@ -280,6 +308,15 @@ class Pipeline:
         """depth/color/normals: device tensors ((H, W) uint16 as int16,
         (H, W) int32 RGBA8, (H, W, 4) float32)."""
         k = intrinsics or self.config.intrinsics
+        if depth is None or len(getattr(depth, "shape", ())) != 2:
+            raise ValueError("depth: must be an (H, W) device image")
+        h, w = int(depth.shape[0]), int(depth.shape[1])
+        if h > self.config.height or w > self.config.width:
+            raise ValueError(f"depth: {w} x {h} exceeds the configured camera {self.config.width} x "
+                             f"{self.config.height}")
+        _check_image(depth, "depth", _DEPTH, (h, w))
+        _check_image(color, "color", _COLOR, (h, w), optional=True)
+        _check_image(normals, "normals", _FLOAT, (h, w, 4), optional=True)
         self._torch.cuda.synchronize()  # inputs come from torch's stream
         f = N.KfDepthFrame()
         f.depth = depth.data_ptr()
@@ -292,6 +329,8 @@ class Pipeline:
         counts = (C.c_int64 * 2)() if count else None
         self.hold(depth, color, normals)
         self._check(N.hip().kf_integrate(self._ctx, C.byref(f), counts))
+        if count:
+            self._inflight.clear()  # the counting call waited for the stream
         return (counts[0], counts[1]) if count else None
 
     # ---- raycast (pipeline.hpp:82) ----------------------------------------
@@ -340,14 +379,21 @@ class Pipeline:
                       sync_inputs: bool = True):
         """report=False only enqueues the frame; inputs produced on torch's
         stream must be complete first (sync_inputs)."""
+        shape = (self.config.height, self.config.width)
+        _check_image(depth, "depth", _DEPTH, shape)
+        _check_image(color, "color", _COLOR, shape, optional=True)
         if sync_inputs:
             self._torch.cuda.synchronize()
         opts = (icp or ProjectiveIcpOptions()).native()
         res = N.KfFrameResult() if report else None
         self.hold(depth, color)
-        self._check(N.hip().kf_process_frame(self._ctx, depth.data_ptr(),
-                                             color.data_ptr() if color is not None else None, C.byref(opts),
-                                             C.byref(res) if report else None))
+        try:
+            self._check(N.hip().kf_process_frame(self._ctx, depth.data_ptr(),
+                                                 color.data_ptr() if color is not None else None, C.byref(opts),
+                                                 C.byref(res) if report else None))
+        finally:
+            if report:
+                self._inflight.clear()  # the reporting call waited for the stream
         return res
 
     def trajectory(self) -> np.ndarray:
@@ -362,6 +408,11 @@ class Pipeline:
         t = N.KfStageTimes()
         self._check(N.hip().kf_last_stage_times(self._ctx, C.byref(t)))
         return t
+
+    def set_sync_timeout(self, ticks: int) -> None:
+        """Bound (device wall-clock ticks, 100 MHz) of the grid-wide wait of
+        the learned fallback pass's one-launch levels; a diagnostic."""
+        self._check(N.hip().kf_set_sync_timeout(self._ctx, int(ticks)))
 
     def attach_posenet(self, net) -> None:
         """Use a trained PoseNet for init="learned" frames (None detaches)."""
@@ -387,6 +438,21 @@ class Pipeline:
         out = (C.c_int64 * 4)()
         self._check(N.hip().kf_hashed_stats(self._ctx, out))
         return {"allocated": out[0], "overflow": out[1], "entries": out[2], "last_work": out[3]}
+
+    def hashed_download(self):
+        """(keys increasing, voxels (n, 512, 2), colours (n, 512, 4) or None)
+        of the allocated blocks of a hashed volume."""
+        n = C.c_int64()
+        self._check(N.hip().kf_hashed_download(self._ctx, None, None, None, 0, C.byref(n)))
+        m = n.value
+        keys = np.empty(max(m, 1), np.uint32)
+        vox = np.empty((max(m, 1), 512, 2), np.float32 if self.config.voxel == "float" else np.uint16)
+        col = np.empty((max(m, 1), 512, 4), np.float32) if self.config.color == "float" else None
+        self._check(N.hip().kf_hashed_download(self._ctx, keys.ctypes.data_as(C.c_void_p),
+                                               vox.ctypes.data_as(C.c_void_p),
+                                               col.ctypes.data_as(C.c_void_p) if col is not None else None, m,
+                                               C.byref(n)))
+        return keys[:m], vox[:m], (col[:m] if col is not None else None)
 
     def hashed_blocks(self) -> np.ndarray:
         """Sorted keys of the allocated blocks (flat x-major block indices)."""
@@ -437,6 +503,8 @@ class PyramidSource:
         self.pipeline = pipeline
 
     def build(self, raw_depth) -> int:
+        c = self.pipeline.config
+        _check_image(raw_depth, "raw_depth", _DEPTH, (c.height, c.width))
         self.pipeline._torch.cuda.synchronize()
         self.pipeline.hold(raw_depth)
         n = C.c_int32()
@@ -458,6 +526,19 @@ class PyramidSource:
         }
 
 
+def _check_maps(live_vertices, live_normals, model_points, model_normals):
+    """(live h, w, model h, w) of float4 maps the ICP kernels read."""
+    if len(getattr(live_vertices, "shape", ())) != 3 or len(getattr(model_points, "shape", ())) != 3:
+        raise ValueError("maps: must be (H, W, 4) device images")
+    lh, lw = int(live_vertices.shape[0]), int(live_vertices.shape[1])
+    mh, mw = int(model_points.shape[0]), int(model_points.shape[1])
+    _check_image(live_vertices, "live_vertices", _FLOAT, (lh, lw, 4))
+    _check_image(live_normals, "live_normals", _FLOAT, (lh, lw, 4))
+    _check_image(model_points, "model_points", _FLOAT, (mh, mw, 4))
+    _check_image(model_normals, "model_normals", _FLOAT, (mh, mw, 4))
+    return lh, lw, mh, mw
+
+
 class ProjectiveIcpTracker:
     """icp_optimizer.hpp:107-130 in its whole-level device-loop form."""
 
@@ -473,9 +554,8 @@ class ProjectiveIcpTracker:
         pose = (C.c_double * 16)(*np.asarray(initial_camera_to_world, np.float64).reshape(16))
         res = N.KfIcpResult()
         opts = self.options.native()
+        lh, lw, mh, mw = _check_maps(live_vertices, live_normals, model_points, model_normals)
         self.pipeline._torch.cuda.synchronize()
-        lh, lw = int(live_vertices.shape[0]), int(live_vertices.shape[1])
-        mh, mw = int(model_points.shape[0]), int(model_points.shape[1])
         N.check(N.hip().kf_icp_track_maps(self.pipeline.ctx, live_vertices.data_ptr(), live_normals.data_ptr(),
                                           lw, lh, model_points.data_ptr(), model_normals.data_ptr(), mw, mh, k,
                                           mp, pose, iterations, C.byref(opts), C.byref(res)), self.pipeline.ctx)
@@ -490,9 +570,8 @@ class ProjectiveIcpTracker:
         mod = (C.c_float * 12)(*np.asarray(model12, np.float32).reshape(12))
         sums = (C.c_double * 29)()
         count = C.c_int64()
+        lh, lw, mh, mw = _check_maps(live_vertices, live_normals, model_points, model_normals)
         self.pipeline._torch.cuda.synchronize()
-        lh, lw = int(live_vertices.shape[0]), int(live_vertices.shape[1])
-        mh, mw = int(model_points.shape[0]), int(model_points.shape[1])
         N.check(N.hip().kf_icp_reduce(self.pipeline.ctx, live_vertices.data_ptr(), live_normals.data_ptr(), lw, lh,
                                       model_points.data_ptr(), model_normals.data_ptr(), mw, mh, k, cam, mod,
                                       self.options.max_point_distance, self.options.min_normal_dot, sums,

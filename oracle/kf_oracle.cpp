@@ -180,9 +180,23 @@ struct Volume {
   int kind;
   void* voxels;
   float* colors;  // 4 floats per voxel (r, g, b, weight) or null
+  // Sparse store (BlockRep, block_rep.hpp:183-307): the pool slot of every
+  // 8^3 block (-1 unallocated, grid.hpp:44-59) and voxels/colors are the
+  // pool, 512 voxels per slot in intra-block order (grid.hpp:61-69); null
+  // for the dense store.
+  const std::int32_t* slots = nullptr;
 
   std::size_t flat(std::int64_t x, std::int64_t y, std::int64_t z) const {  // vector.hpp:62-68
     return static_cast<std::size_t>(((z * ny) + y) * nx + x);
+  }
+  // Storage index of voxel (x, y, z), or -1 when its block is unallocated
+  // (SparseVolumeView::pool_at returns null, block_rep.hpp:87-100).
+  std::int64_t index(std::int64_t x, std::int64_t y, std::int64_t z) const {
+    if (slots == nullptr) return static_cast<std::int64_t>(flat(x, y, z));
+    const std::int64_t bx = (nx + 7) / 8, by = (ny + 7) / 8;
+    const std::int32_t slot = slots[((z / 8) * by + (y / 8)) * bx + (x / 8)];
+    if (slot < 0) return -1;
+    return static_cast<std::int64_t>(slot) * 512 + (((z % 8) * 8 + (y % 8)) * 8 + (x % 8));
   }
   V3 cell_center(std::int64_t x, std::int64_t y, std::int64_t z) const {  // volume.hpp:276-283
     const V3 start = mk(static_cast<float>(x), static_cast<float>(y), static_cast<float>(z));
@@ -251,7 +265,7 @@ struct Sampler {
     const long long y = static_cast<long long>(s.by) + oy;
     const long long z = static_cast<long long>(s.bz) + oz;
     if (!vol->contains_i(x, y, z)) return -1;
-    return static_cast<long long>(vol->flat(x, y, z));
+    return static_cast<long long>(vol->index(x, y, z));
   }
 
   // TrilinearSampler::tsdf + blend_tsdf (volume_sampler.hpp:141-158,
@@ -424,15 +438,17 @@ struct kfo_integration_options {
   float truncation_distance_scale;
 };
 
-static Volume make_volume(const kfo_geometry* g, int kind, const void* voxels, const float* colors) {
+static Volume make_volume(const kfo_geometry* g, int kind, const void* voxels, const float* colors,
+                          const std::int32_t* slots = nullptr) {
   return Volume{g->nx, g->ny, g->nz, g->voxel_size, mk(g->ox, g->oy, g->oz), g->truncation,
-                kind, const_cast<void*>(voxels), const_cast<float*>(colors)};
+                kind, const_cast<void*>(voxels), const_cast<float*>(colors), slots};
 }
 
 // Fresh voxels: tsdf = kUnobservedTsdf, weight 0 (volume.hpp:23,107-108);
-// colour voxels zero (volume.hpp:169-171).
-void kfo_volume_reset(const kfo_geometry* g, int kind, void* voxels, float* colors) {
-  const std::size_t n = static_cast<std::size_t>(g->nx * g->ny * g->nz);
+// colour voxels zero (volume.hpp:169-171).  n voxels (a dense grid, or a
+// sparse pool's slots x 512).
+void kfo_voxels_reset(int kind, void* voxels, float* colors, std::int64_t count) {
+  const std::size_t n = static_cast<std::size_t>(count);
   if (kind == kFloatVoxel) {
     auto* v = static_cast<FloatVoxel*>(voxels);
     for (std::size_t i = 0; i < n; ++i) v[i] = FloatVoxel{1.0F, 0.0F};
@@ -443,6 +459,10 @@ void kfo_volume_reset(const kfo_geometry* g, int kind, void* voxels, float* colo
     for (std::size_t i = 0; i < n; ++i) v[i] = PackedVoxel{d, 0};
   }
   if (colors != nullptr) std::memset(colors, 0, n * 4 * sizeof(float));
+}
+
+void kfo_volume_reset(const kfo_geometry* g, int kind, void* voxels, float* colors) {
+  kfo_voxels_reset(kind, voxels, colors, g->nx * g->ny * g->nz);
 }
 
 // IntegrationContext::observe/fuse (tsdf_integration.hpp:183-252) with the
@@ -483,7 +503,9 @@ static int integrate_voxel(Volume& vol, std::int64_t x, std::int64_t y, std::int
   }
   // fuse (:230-252)
   if (w <= 0.0F) return 0;
-  const std::size_t i = vol.flat(x, y, z);
+  const std::int64_t at = vol.index(x, y, z);
+  if (at < 0) return 0;  // sparse: only allocated blocks are integrated
+  const std::size_t i = static_cast<std::size_t>(at);
   vol.fuse(i, tsdf, w, opt->max_weight);
   if (vol.colors != nullptr && color != nullptr && integrate_color) {
     const V3 c = color_from_pixel(color[static_cast<std::size_t>(row) * width + col]);
@@ -528,11 +550,11 @@ int kfo_integrate(const kfo_geometry* g, int kind, int rule, void* voxels, float
 
 // ---------------------------------------------------------------------------
 // Sparse (block) volume: BlockRep<kHost> (block_rep.hpp:183-307) driven by
-// TruncationBandWalk (tsdf_integration.hpp:378-437).  The volume is kept as
-// its dense expansion (an unallocated block reads as fresh voxels, which every
-// reader treats like the reference's missing corners: trilinear.hpp:150-160)
-// plus the reference's slot grid, so readers compare one to one; the pool
-// layout itself is not observable through the reference's interfaces.
+// TruncationBandWalk (tsdf_integration.hpp:378-437).  The volume is the
+// reference's slot grid (one int32 per 8^3 block, -1 unallocated) plus a
+// pool of 512-voxel slots in allocation order; every reader addresses a
+// voxel through Volume::index, and a voxel of an unallocated block is a
+// missing corner (trilinear.hpp:150-160), as in the reference.
 // ---------------------------------------------------------------------------
 namespace {
 constexpr std::int64_t kBlockEdge = 8;  // grid.hpp:15-17
@@ -603,13 +625,14 @@ std::int64_t kfo_band_walk(const kfo_geometry* g, const std::uint16_t* depth, in
 // block, allocation in walk order (slots[block] = pool slot or -1;
 // state[0] = blocks allocated so far including overflowing requests,
 // state[1] = capacity), the frame's work list once per block, then the
-// listed blocks' voxels updated as in the dense sweep.  counts as in
-// kfo_integrate plus counts[2] = blocks in the work list.
+// listed blocks' voxels updated as in the dense sweep.  voxels/colors are
+// the pool (capacity x 512 voxels).  counts as in kfo_integrate plus
+// counts[2] = blocks in the work list.
 int kfo_integrate_sparse(const kfo_geometry* g, int kind, int rule, void* voxels, float* colors,
                          std::int32_t* slots, std::int64_t* state, const std::uint16_t* depth, int width,
                          int height, const std::uint32_t* color, const float* normals, const kfo_intrinsics* k,
                          const float* world_to_camera, const kfo_integration_options* opt, std::int64_t* counts) {
-  Volume vol = make_volume(g, kind, voxels, colors);
+  Volume vol = make_volume(g, kind, voxels, colors, slots);
   const BlockDims blocks = block_dims(vol);
   const float step = 0.5F * static_cast<float>(kBlockEdge) * vol.voxel_size;  // :380-383
   std::vector<std::int64_t> visits(static_cast<std::size_t>(width) * height * 8);
@@ -655,6 +678,49 @@ int kfo_integrate_sparse(const kfo_geometry* g, int kind, int rule, void* voxels
     counts[0] = fused;
     counts[1] = color_fused;
     counts[2] = static_cast<std::int64_t>(work.size());
+  }
+  return 0;
+}
+
+// BandIntegrationSweep<kHost>::run (tsdf_integration.hpp:444-497), the
+// dense kBand mode: the band walk at a one-voxel step (:462-463) marks a
+// block bitmap, the set bits are listed in increasing block order, and the
+// listed blocks' voxels are updated as in the full sweep; every other voxel
+// keeps its value.  counts as kfo_integrate plus counts[2] = blocks listed.
+int kfo_integrate_band(const kfo_geometry* g, int kind, int rule, void* voxels, float* colors,
+                       const std::uint16_t* depth, int width, int height, const std::uint32_t* color,
+                       const float* normals, const kfo_intrinsics* k, const float* world_to_camera,
+                       const kfo_integration_options* opt, std::int64_t* counts) {
+  Volume vol = make_volume(g, kind, voxels, colors);
+  const BlockDims blocks = block_dims(vol);
+  std::vector<std::int64_t> visits(static_cast<std::size_t>(width) * height * 16);
+  std::int64_t n = kfo_band_walk(g, depth, width, height, k, world_to_camera, opt, vol.voxel_size, visits.data(),
+                                 static_cast<std::int64_t>(visits.size()));
+  if (n > static_cast<std::int64_t>(visits.size())) {
+    visits.resize(static_cast<std::size_t>(n));
+    n = kfo_band_walk(g, depth, width, height, k, world_to_camera, opt, vol.voxel_size, visits.data(), n);
+  }
+  std::vector<std::uint8_t> bitmap(static_cast<std::size_t>(blocks.count()), 0);
+  for (std::int64_t i = 0; i < n; ++i) bitmap[static_cast<std::size_t>(visits[static_cast<std::size_t>(i)])] = 1;
+  const Rigid w2c = rigid_from(world_to_camera);
+  const Intr intr{k->fx, k->fy, k->cx, k->cy};
+  std::int64_t fused = 0, color_fused = 0, listed = 0;
+  for (std::int64_t b = 0; b < blocks.count(); ++b) {
+    if (bitmap[static_cast<std::size_t>(b)] == 0) continue;
+    ++listed;
+    const std::int64_t bx = b % blocks.bx, by = (b / blocks.bx) % blocks.by, bz = b / (blocks.bx * blocks.by);
+    for (std::int64_t z = bz * kBlockEdge; z < cmin(vol.nz, (bz + 1) * kBlockEdge); ++z)
+      for (std::int64_t y = by * kBlockEdge; y < cmin(vol.ny, (by + 1) * kBlockEdge); ++y)
+        for (std::int64_t x = bx * kBlockEdge; x < cmin(vol.nx, (bx + 1) * kBlockEdge); ++x) {
+          const int r = integrate_voxel(vol, x, y, z, rule, w2c, intr, depth, width, height, color, normals, opt);
+          fused += r > 0 ? 1 : 0;
+          color_fused += r > 1 ? 1 : 0;
+        }
+  }
+  if (counts != nullptr) {
+    counts[0] = fused;
+    counts[1] = color_fused;
+    counts[2] = listed;
   }
   return 0;
 }
@@ -776,11 +842,11 @@ void kfo_normals(const float* vertices, int w, int h, float max_jump, float* out
 // taken (the march work) in *samples when non-null.
 // ---------------------------------------------------------------------------
 int kfo_raycast(const kfo_geometry* g, int kind, const void* voxels, const float* colors,
-                const float* cam_to_world, const kfo_intrinsics* k, int width, int height,
-                float min_depth, float max_depth, float step_scale, int require_all,
-                int cell_normals, float* points, float* normals_out, std::uint32_t* colors_out,
-                std::int64_t* samples) {
-  Volume vol = make_volume(g, kind, voxels, colors);
+                const std::int32_t* slots, const float* cam_to_world, const kfo_intrinsics* k, int width,
+                int height, float min_depth, float max_depth, float step_scale, int require_all,
+                int cell_normals, const float* previous_points, float* points, float* normals_out,
+                std::uint32_t* colors_out, std::int64_t* samples) {
+  Volume vol = make_volume(g, kind, voxels, colors, slots);
   const Sampler smp(&vol);
   const Rigid pose = rigid_from(cam_to_world);
   const Intr intr{k->fx, k->fy, k->cx, k->cy};
@@ -789,6 +855,16 @@ int kfo_raycast(const kfo_geometry* g, int kind, const void* voxels, const float
   for (int row = 0; row < height; ++row)
     for (int col = 0; col < width; ++col) {
       const std::size_t pi = static_cast<std::size_t>(row) * width + col;
+      // seed_from_previous (raycasting.hpp:122-131): start two truncation
+      // bands in front of this pixel's previous surface point
+      float seed = min_depth;
+      if (previous_points != nullptr) {
+        const V3 prev = mk(previous_points[3 * pi], previous_points[3 * pi + 1], previous_points[3 * pi + 2]);
+        if (all_finite(prev) && dot(prev, prev) > 0.0F) {
+          const float previous_depth = norm(prev - pose.t);
+          seed = cmax(min_depth, previous_depth - (2.0F * vol.truncation));
+        }
+      }
       V3 P = nan3(), N = nan3();
       std::uint32_t C = 0;
       const V3 translated = back_project(intr, static_cast<float>(col), static_cast<float>(row), 1.0F);
@@ -800,7 +876,7 @@ int kfo_raycast(const kfo_geometry* g, int kind, const void* voxels, const float
         bool have_prev = false, hit = false;
         float prev_t = 0.0F;
         V3 prev_p{}, surface{};
-        float len = min_depth;
+        float len = seed;
         while (len <= max_depth) {
           const V3 point = pose.t + (len * dir);
           float t;
@@ -1240,7 +1316,7 @@ int edge_of(char c) { return c <= '9' ? c - '0' : c - 'a' + 10; }
 void* kfo_mesh_extract_traversal(const kfo_geometry* g, int kind, const void* voxels, const float* colors,
                                  float min_weight, const std::int32_t* slots) {
   auto* mesh = new kfo_mesh();
-  Volume vol = make_volume(g, kind, voxels, colors);
+  Volume vol = make_volume(g, kind, voxels, colors, slots);
   if (vol.nx < 2 || vol.ny < 2 || vol.nz < 2) return mesh;
   const Sampler smp(&vol);
   std::unordered_map<std::size_t, std::uint32_t> welded;
@@ -1249,7 +1325,9 @@ void* kfo_mesh_extract_traversal(const kfo_geometry* g, int kind, const void* vo
         float val[8];
         unsigned cube = 0;
         for (int c = 0; c < 8; ++c) {  // add_cell (:367-387)
-          const Loaded v = vol.load(vol.flat(cx + kCorner[c][0], cy + kCorner[c][1], cz + kCorner[c][2]));
+          const std::int64_t at = vol.index(cx + kCorner[c][0], cy + kCorner[c][1], cz + kCorner[c][2]);
+          if (at < 0) return;  // a corner in an unallocated block is missing
+          const Loaded v = vol.load(static_cast<std::size_t>(at));
           if (!v.observed || !v.finite || v.weight < min_weight) return;
           val[c] = v.tsdf;
           if (val[c] < 0.0F) cube |= 1U << c;

@@ -98,6 +98,9 @@ def _p(a):
 def _declare(L: C.CDLL) -> None:
     vp, i32, i64, f32 = C.c_void_p, C.c_int, C.c_int64, C.c_float
     L.kfo_volume_reset.argtypes = [C.POINTER(Geometry), i32, vp, vp]
+    L.kfo_voxels_reset.argtypes = [i32, vp, vp, i64]
+    L.kfo_integrate_band.argtypes = [C.POINTER(Geometry), i32, i32, vp, vp, vp, i32, i32, vp, vp,
+                                     C.POINTER(Intrinsics), vp, C.POINTER(IntegrationOptions), vp]
     L.kfo_integrate.argtypes = [C.POINTER(Geometry), i32, i32, vp, vp, vp, i32, i32, vp, vp,
                                 C.POINTER(Intrinsics), vp, C.POINTER(IntegrationOptions), vp]
     L.kfo_observed_count.argtypes = [C.POINTER(Geometry), i32, vp]
@@ -106,8 +109,8 @@ def _declare(L: C.CDLL) -> None:
     L.kfo_downsample.argtypes = [vp, i32, i32, f32, f32, vp]
     L.kfo_vertices.argtypes = [vp, i32, i32, C.POINTER(Intrinsics), f32, f32, f32, vp]
     L.kfo_normals.argtypes = [vp, i32, i32, f32, vp]
-    L.kfo_raycast.argtypes = [C.POINTER(Geometry), i32, vp, vp, vp, C.POINTER(Intrinsics), i32, i32,
-                              f32, f32, f32, i32, i32, vp, vp, vp, vp]
+    L.kfo_raycast.argtypes = [C.POINTER(Geometry), i32, vp, vp, vp, vp, C.POINTER(Intrinsics), i32, i32,
+                              f32, f32, f32, i32, i32, vp, vp, vp, vp, vp]
     L.kfo_icp_sums.argtypes = [vp, vp, i32, i32, vp, vp, i32, i32, C.POINTER(Intrinsics), vp, vp, f32, f32,
                                vp, vp]
     L.kfo_icp_sums.restype = i64
@@ -205,21 +208,54 @@ def integrate(vol: Volume, rule: int, depth: np.ndarray, intr, world_to_camera, 
 
 
 class SparseVolume(Volume):
-    """BlockRep<kHost> (block_rep.hpp:183-307) kept as its dense expansion
-    plus the slot grid (slots[block] = pool slot or -1) and the allocation
-    state [requested, capacity]."""
+    """BlockRep<kHost> (block_rep.hpp:183-307): the slot grid (slots[block]
+    = pool slot or -1, flat x-major block order), the allocation state
+    [requested, capacity] and a pool of capacity x 512 voxels (`voxels`,
+    `colors`), each slot's voxels in intra-block order
+    (z % 8 * 8 + y % 8) * 8 + x % 8."""
     slots: np.ndarray
     state: np.ndarray
 
     @staticmethod
     def create_sparse(res, voxel_size, origin, truncation, kind=FLOAT, color=True, capacity=None) -> "SparseVolume":
-        base = Volume.create(res, voxel_size, origin, truncation, kind, color)
-        g = base.geometry
-        nb = ((g.nx + 7) // 8) * ((g.ny + 7) // 8) * ((g.nz + 7) // 8)
-        v = SparseVolume(base.geometry, base.kind, base.voxels, base.colors)
+        nx, ny, nz = (res, res, res) if np.isscalar(res) else res
+        g = Geometry(nx, ny, nz, voxel_size, origin[0], origin[1], origin[2], truncation)
+        nb = ((nx + 7) // 8) * ((ny + 7) // 8) * ((nz + 7) // 8)
+        cap = capacity if capacity is not None else nb // 4
+        n = cap * 512
+        vox = np.empty((n, 2), np.float32 if kind == FLOAT else np.uint16)
+        col = np.empty((n, 4), np.float32) if color else None
+        lib().kfo_voxels_reset(kind, _p(vox), _p(col), n)
+        v = SparseVolume(g, kind, vox, col)
         v.slots = np.full(nb, -1, np.int32)
-        v.state = np.array([0, capacity if capacity is not None else nb // 4], np.int64)
+        v.state = np.array([0, cap], np.int64)
         return v
+
+    def blocks(self) -> tuple[np.ndarray, np.ndarray, np.ndarray | None]:
+        """(keys increasing, voxels (n, 512, 2), colours (n, 512, 4) or None)
+        of the allocated blocks."""
+        keys = np.nonzero(self.slots >= 0)[0]
+        slot = self.slots[keys].astype(np.int64)
+        vox = self.voxels.reshape(-1, 512, 2)[slot]
+        col = self.colors.reshape(-1, 512, 4)[slot] if self.colors is not None else None
+        return keys.astype(np.uint32), vox, col
+
+    def dense(self) -> Volume:
+        """The dense expansion (BlockRep::host_dense_view): unallocated
+        voxels fresh.  For small grids only."""
+        g = self.geometry
+        out = Volume.create((g.nx, g.ny, g.nz), g.voxel_size, (g.ox, g.oy, g.oz), g.truncation, self.kind,
+                            self.colors is not None)
+        idx = np.arange(g.nx * g.ny * g.nz)
+        x, y, z = idx % g.nx, (idx // g.nx) % g.ny, idx // (g.nx * g.ny)
+        bx, by = (g.nx + 7) // 8, (g.ny + 7) // 8
+        slot = self.slots[((z // 8) * by + (y // 8)) * bx + (x // 8)].astype(np.int64)
+        have = slot >= 0
+        at = slot[have] * 512 + ((z[have] % 8) * 8 + (y[have] % 8)) * 8 + (x[have] % 8)
+        out.voxels[have] = self.voxels[at]
+        if out.colors is not None:
+            out.colors[have] = self.colors[at]
+        return out
 
 
 def integrate_sparse(vol: SparseVolume, rule: int, depth: np.ndarray, intr, world_to_camera, opts=None,
@@ -255,7 +291,29 @@ def band_walk(geometry: Geometry, depth: np.ndarray, intr, world_to_camera, opts
     return out[:n]
 
 
+def integrate_band(vol: Volume, rule: int, depth: np.ndarray, intr, world_to_camera, opts=None,
+                   color: np.ndarray | None = None, normals: np.ndarray | None = None) -> tuple[int, int, int]:
+    """BandIntegrationSweep<kHost> (dense kBand mode): (voxels fused, colour
+    voxels fused, blocks listed)."""
+    h, w = depth.shape
+    depth = np.ascontiguousarray(depth, np.uint16)
+    if color is not None:
+        color = np.ascontiguousarray(color, np.uint32)
+    if normals is not None:
+        normals = np.ascontiguousarray(normals[..., :3], np.float32)
+    opts = opts or integration_options()
+    counts = np.zeros(3, np.int64)
+    lib().kfo_integrate_band(C.byref(vol.geometry), vol.kind, rule, _p(vol.voxels), _p(vol.colors), _p(depth), w, h,
+                             _p(color), _p(normals), C.byref(Intrinsics(*intr)), _p(pose12(world_to_camera)),
+                             C.byref(opts), _p(counts))
+    return int(counts[0]), int(counts[1]), int(counts[2])
+
+
 def observed_count(vol: Volume) -> int:
+    if isinstance(vol, SparseVolume):
+        keys, vox, _ = vol.blocks()
+        w = vox[..., 1]
+        return int(np.count_nonzero(w > 0) if vol.kind == FLOAT else np.count_nonzero(w != 0))
     return int(lib().kfo_observed_count(C.byref(vol.geometry), vol.kind, _p(vol.voxels)))
 
 
@@ -310,15 +368,20 @@ def pyramid(depth, intr, scale, levels=3, lo=0.4, hi=8.0, radius=2, sigma_s=2.0,
 
 
 def raycast(vol: Volume, cam_to_world, intr, width, height, min_depth=0.4, max_depth=8.0, step_scale=1.0,
-            require_all=False, cell_normals=False):
+            require_all=False, cell_normals=False, previous_points=None):
+    """SurfaceRaycast over every pixel: (points, normals, colours, samples).
+    previous_points ((H, W, 3) float32): seed_from_previous with these as the
+    maps' previous surface points."""
     pts = np.empty((height, width, 3), np.float32)
     nrm = np.empty((height, width, 3), np.float32)
     col = np.empty((height, width), np.uint32)
     samples = np.zeros(1, np.int64)
     xf = pose12(cam_to_world)
-    lib().kfo_raycast(C.byref(vol.geometry), vol.kind, _p(vol.voxels), _p(vol.colors), _p(xf),
+    prev = None if previous_points is None else np.ascontiguousarray(previous_points[..., :3], np.float32)
+    slots = vol.slots if isinstance(vol, SparseVolume) else None
+    lib().kfo_raycast(C.byref(vol.geometry), vol.kind, _p(vol.voxels), _p(vol.colors), _p(slots), _p(xf),
                       C.byref(Intrinsics(*intr)), width, height, min_depth, max_depth, step_scale,
-                      int(require_all), int(cell_normals), _p(pts), _p(nrm), _p(col), _p(samples))
+                      int(require_all), int(cell_normals), _p(prev), _p(pts), _p(nrm), _p(col), _p(samples))
     return pts, nrm, col, int(samples[0])
 
 

@@ -52,3 +52,33 @@ def test_cpu_space_is_refused():
 
     with pytest.raises(NotImplementedError):
         K.PipelineConfig(space="cpu").native()
+
+
+def test_device_images_are_validated_before_any_pointer_crosses():
+    """Host tensors, strided views, wrong dtypes and wrong shapes are refused
+    with ValueError before their data pointer reaches the library."""
+    import pytest
+    import torch
+
+    from kinectfusion_amd.pipeline import _check_image
+
+    cpu = torch.zeros((4, 6), dtype=torch.int16)
+    with pytest.raises(ValueError, match="device"):
+        _check_image(cpu, "depth", ("int16", "uint16"), (4, 6))
+
+    class FakeCuda:  # a device tensor's attributes, without a device
+        def __init__(self, t, contiguous=True):
+            self.t, self.is_cuda, self._c = t, True, contiguous
+            self.dtype, self.shape = t.dtype, t.shape
+
+        def is_contiguous(self):
+            return self._c
+
+    with pytest.raises(ValueError, match="contiguous"):
+        _check_image(FakeCuda(cpu, contiguous=False), "depth", ("int16",), (4, 6))
+    with pytest.raises(ValueError, match="dtype"):
+        _check_image(FakeCuda(cpu.float()), "depth", ("int16",), (4, 6))
+    with pytest.raises(ValueError, match="shape"):
+        _check_image(FakeCuda(cpu), "depth", ("int16",), (6, 4))
+    assert _check_image(FakeCuda(cpu), "depth", ("int16",), (4, 6)) is not None
+    assert _check_image(None, "color", ("int32",), (4, 6), optional=True) is None

@@ -63,9 +63,13 @@ def test_hashed_integration_bit_exact(K, ko, scene, torch_gpu, store):
     st = p.hashed_stats()
     assert st["allocated"] == int(vol.state[0]) and st["overflow"] == 0
     vox, col = p.host_view()
-    assert np.array_equal(vox.view(np.uint32), vol.voxels.view(np.uint32)), first_mismatch(vox, vol.voxels)
-    assert same_bits(col, vol.colors)
+    ex = vol.dense()
+    assert np.array_equal(vox.view(np.uint32), ex.voxels.view(np.uint32)), first_mismatch(vox, ex.voxels)
+    assert same_bits(col, ex.colors)
     assert p.observed_voxel_count() == ko.observed_count(vol)
+    gk, gv, gc = p.hashed_download()
+    rk, rv, rc = vol.blocks()
+    assert np.array_equal(gk, rk) and same_bits(gv, rv) and same_bits(gc, rc)
     p.close()
 
 
@@ -114,7 +118,7 @@ def test_exhausted_pool_writes_only_allocated_blocks(K, ko, scene, torch_gpu):
     x, y, z = idx % g.nx, (idx // g.nx) % g.ny, idx // (g.nx * g.ny)
     block = ((z // 8) * by + (y // 8)) * bx + (x // 8)
     inside = np.isin(block, keys)
-    assert np.array_equal(vox[inside], full.voxels[inside])
+    assert np.array_equal(vox[inside], full.dense().voxels[inside])
     assert np.all(vox[~inside, 1] == 0.0) and np.all(vox[~inside, 0] == 1.0)
     p.close()
 

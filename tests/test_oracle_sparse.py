@@ -44,8 +44,9 @@ def test_sparse_equals_dense_on_allocated_blocks(ko):
     fused, _, listed = ko.integrate_sparse(sp, 0, depth, small_intr(), pose, opts)
     assert listed > 0 and fused > 0
     alloc = np.isin(blocks_of(sp), np.nonzero(sp.slots >= 0)[0])
-    assert np.array_equal(sp.voxels[alloc], dn.voxels[alloc])
-    assert np.all(sp.voxels[~alloc, 1] == 0.0)
+    ex = sp.dense()
+    assert np.array_equal(ex.voxels[alloc], dn.voxels[alloc])
+    assert np.all(ex.voxels[~alloc, 1] == 0.0)
     # the band (|sd| < truncation) lies in the allocated blocks, up to the
     # block corners that half-block steps can pass between
     t, w = dn.decoded()
@@ -63,8 +64,9 @@ def test_exhausted_pool_drops_blocks_in_walk_order(ko):
     assert np.count_nonzero(sp.slots >= 0) == 1 and sp.slots[visits[0]] == 0
     assert sp.state[0] > sp.state[1]  # overflow recorded
     alloc = blocks_of(sp) == visits[0]
-    assert np.array_equal(sp.voxels[alloc], dn.voxels[alloc])
-    assert np.all(sp.voxels[~alloc, 1] == 0.0)
+    ex = sp.dense()
+    assert np.array_equal(ex.voxels[alloc], dn.voxels[alloc])
+    assert np.all(ex.voxels[~alloc, 1] == 0.0)
 
 
 def test_band_samples_stay_within_the_band_of_a_plane(ko):
@@ -88,7 +90,7 @@ def test_sparse_mesh_is_the_dense_mesh_reordered(ko, scene_room):
     for i in range(2):
         depth, rgba, pose = scene_room.frame(i)
         ko.integrate_sparse(vol, 1, depth, INTR, ko.rigid_inverse(pose), ko.integration_options(1000.0), color=rgba)
-    dense = ko.Volume(vol.geometry, vol.kind, vol.voxels, vol.colors)
+    dense = vol.dense()
 
     def tri_set(m):
         pos, _, _, tri = m
