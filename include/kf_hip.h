@@ -293,6 +293,28 @@ kf_status kf_icp_reduce(kf_context* ctx, const float* live_vertices, const float
                         float max_point_distance, float min_normal_dot, double* sums,
                         int64_t* count);
 
+/* Allocation-free form of kf_icp_reduce.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It takes
+ * no context and calls no allocator: the sweep's partial rows (at most 256
+ * rows of 32 floats, 32 KiB) live in a static array of the library that each
+ * device holds until the process ends.  Live and model maps of at most
+ * KF_ICP_STATIC_MAX_PIXELS pixels each; larger maps are rejected
+ * (KF_ERR_INVALID_ARGUMENT).  Inputs as kf_icp_reduce (maps on the device,
+ * model_intrinsics / camera / model on the host); outputs on the DEVICE:
+ * sums (29 doubles, the FP32 compensated sums widened) and count (one
+ * int64).  stream: a hipStream_t, or NULL for the default stream; the call
+ * only enqueues.  Every call on one device shares that device's rows, so
+ * calls that overlap on one device (other streams, other threads) race;
+ * each call fully writes the rows it reads, so nothing needs clearing.
+ * Failures are described by kf_last_error(NULL). */
+#define KF_ICP_STATIC_MAX_PIXELS (1 << 24)
+kf_status kf_icp_reduce_static(const float* live_vertices, const float* live_normals, int32_t live_width,
+                               int32_t live_height, const float* model_points, const float* model_normals,
+                               int32_t model_width, int32_t model_height, const float* model_intrinsics,
+                               const float* camera, const float* model, float max_point_distance,
+                               float min_normal_dot, double* sums, int64_t* count, void* stream);
+
 /* ProjectiveIcpTracker::estimate_pose in its whole-level device-loop form
  * (icp_optimizer.cpp:82-169, cuda/icp_correspondence.cu:216-343), with the
  * 6x6 Cholesky solve and SE(3) composition in FP64 on the device and one

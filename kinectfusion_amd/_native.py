@@ -13,6 +13,7 @@ HIP_LIB = PKG / "libkf_hip.so"
 SCENE_LIB = PKG / "libkf_scene.so"
 
 KF_OK = 0
+KF_ERR_INVALID_ARGUMENT = 1
 STATUS_NAMES = {0: "KF_OK", 1: "KF_ERR_INVALID_ARGUMENT", 2: "KF_ERR_OUT_OF_MEMORY", 3: "KF_ERR_DEVICE",
                 4: "KF_ERR_UNSUPPORTED", 5: "KF_ERR_STATE"}
 
@@ -121,6 +122,8 @@ HIP_SYMBOLS = [
     ("kf_raycast", _i32, [_vp, _dp, _i32, C.POINTER(KfMapsView)]),
     ("kf_icp_reduce", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _i32, C.POINTER(_f32), C.POINTER(_f32),
                              C.POINTER(_f32), _f32, _f32, _dp, C.POINTER(_i64)]),
+    ("kf_icp_reduce_static", _i32, [_vp, _vp, _i32, _i32, _vp, _vp, _i32, _i32, C.POINTER(_f32), C.POINTER(_f32),
+                                    C.POINTER(_f32), _f32, _f32, _vp, _vp, _vp]),
     ("kf_icp_track_maps", _i32, [_vp, _vp, _vp, _i32, _i32, _vp, _vp, _i32, _i32, C.POINTER(_f32), _dp, _dp, _i32,
                                  C.POINTER(KfIcpOptions), C.POINTER(KfIcpResult)]),
     ("kf_track", _i32, [_vp, C.POINTER(KfIcpOptions), _dp, C.POINTER(KfIcpResult)]),

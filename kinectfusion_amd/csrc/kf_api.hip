@@ -790,6 +790,24 @@ kf_status kf_icp_reduce(kf_context* ctx, const float* lv, const float* ln, int32
   });
 }
 
+kf_status kf_icp_reduce_static(const float* lv, const float* ln, int32_t lw, int32_t lh, const float* mv,
+                               const float* mn, int32_t mw, int32_t mh, const float* k, const float* camera,
+                               const float* model, float max_dist, float min_dot, double* sums, int64_t* count,
+                               void* stream) {
+  return guard(nullptr, [&] {
+    require(lv && ln && mv && mn && k && camera && model && sums && count, "kf_icp_reduce_static: null argument");
+    require(lw > 0 && lh > 0 && mw > 0 && mh > 0, "kf_icp_reduce_static: empty maps");
+    require(static_cast<int64_t>(lw) * lh <= KF_ICP_STATIC_MAX_PIXELS,
+            "kf_icp_reduce_static: the live maps exceed KF_ICP_STATIC_MAX_PIXELS pixels");
+    require(static_cast<int64_t>(mw) * mh <= KF_ICP_STATIC_MAX_PIXELS,
+            "kf_icp_reduce_static: the model maps exceed KF_ICP_STATIC_MAX_PIXELS pixels");
+    launch_icp_reduce_static(static_cast<hipStream_t>(stream), reinterpret_cast<const float4*>(lv),
+                             reinterpret_cast<const float4*>(ln), lw, lh, reinterpret_cast<const float4*>(mv),
+                             reinterpret_cast<const float4*>(mn), mw, mh, Intr{k[0], k[1], k[2], k[3]}, camera,
+                             model, max_dist, min_dot, sums, count);
+  });
+}
+
 kf_status kf_icp_track_maps(kf_context* ctx, const float* lv, const float* ln, int32_t lw, int32_t lh,
                             const float* mv, const float* mn, int32_t mw, int32_t mh, const float* k,
                             const double* model_pose, double* pose_inout, int32_t iterations,

@@ -26,7 +26,16 @@
 // stability spectrum (FP64 Jacobi) is checked once per level, by the level's
 // last trial, as in the reference's device loop.  A pass that usually exits
 // at once runs each level in one launch instead (icp_level_kernel).
+#include <mutex>
+
 #include "kf_internal.hpp"
+
+// Partial rows of the allocation-free reduction (kf_icp_reduce_static): one
+// row of 32 floats per sweep workgroup, at most 256 workgroups (32 KiB).
+// Each device holds its own copy for the life of the process; host code
+// reaches it only through the address hipGetSymbolAddress returns for the
+// current device (static_scratch_rows below), never through this name.
+static __device__ __align__(256) float scratch[256 * 32];
 
 namespace kf {
 namespace {
@@ -589,6 +598,18 @@ __global__ void __launch_bounds__(256) icp_combine_kernel(IcpState* st, const fl
   if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
 }
 
+// The same reduction into caller memory: the 29 sums widened to FP64 and the
+// correspondence count (kf_icp_reduce_static).
+__global__ void __launch_bounds__(256) icp_combine_out_kernel(const float* __restrict__ partial, int nblocks,
+                                                               double* __restrict__ sums,
+                                                               int64_t* __restrict__ count) {
+  __shared__ float combined[kSums + 1];
+  combine_partials(partial, nblocks, combined);
+  __syncthreads();
+  if (threadIdx.x < kSums) sums[threadIdx.x] = static_cast<double>(combined[threadIdx.x]);
+  else if (threadIdx.x == kSums) *count = static_cast<int64_t>(__float_as_uint(combined[kSums]));
+}
+
 // FP64 cyclic Jacobi eigenvalues of the symmetric 6x6 system.
 __device__ void sym_eig6(const double* eq, double* ev) {
   double a[6][6];
@@ -853,6 +874,41 @@ void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* l
   hipLaunchKernelGGL(icp_sweep_kernel, dim3(nb), dim3(256), 0, ctx->stream, m,
                      rigid_from12(camera12), rigid_from12(model12), max_dist, min_dot, ctx->partials);
   hipLaunchKernelGGL(icp_combine_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->icp, ctx->partials, nb);
+  KF_HIP(hipGetLastError());
+}
+
+static_assert(sizeof(scratch) == sizeof(float) * kMaxBlocks * kPartialStride,
+              "the static partial rows hold one row per sweep workgroup");
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept.
+float* static_scratch_rows() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static float* rows[kMaxDevices] = {};
+  int dev = 0;
+  KF_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(KF_ERR_UNSUPPORTED, "kf_icp_reduce_static: device index out of range");
+  const std::lock_guard<std::mutex> lock(mu);
+  if (rows[dev] == nullptr) {
+    void* p = nullptr;
+    KF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) fail(KF_ERR_DEVICE, "kf_icp_reduce_static: no device address for the partial rows");
+    rows[dev] = static_cast<float*>(p);
+  }
+  return rows[dev];
+}
+
+void launch_icp_reduce_static(hipStream_t stream, const float4* lv_v, const float4* lv_n, int lw, int lh,
+                              const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
+                              const float* camera12, const float* model12, float max_dist, float min_dot,
+                              double* sums, int64_t* count) {
+  float* rows = static_scratch_rows();
+  const Maps m = make_maps(lv_v, lv_n, lw, lh, md_v, md_n, mw, mh, k);
+  const int nb = blocks_for(lw * lh);  // <= kMaxBlocks rows, every one written by the sweep
+  hipLaunchKernelGGL(icp_sweep_kernel, dim3(nb), dim3(256), 0, stream, m, rigid_from12(camera12),
+                     rigid_from12(model12), max_dist, min_dot, rows);
+  hipLaunchKernelGGL(icp_combine_out_kernel, dim3(1), dim3(256), 0, stream, rows, nb, sums, count);
   KF_HIP(hipGetLastError());
 }
 

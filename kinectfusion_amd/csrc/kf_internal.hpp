@@ -514,6 +514,12 @@ void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* l
                             const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
                             const float* camera12, const float* model12, float max_dist,
                             float min_dot);
+// kf_icp_reduce without a context: the partial rows in the library's static
+// per-device array; sums (29 FP64) and count (device) written by the device.
+void launch_icp_reduce_static(hipStream_t stream, const float4* lv_v, const float4* lv_n, int lw, int lh,
+                              const float4* md_v, const float4* md_n, int mw, int mh, Intr k,
+                              const float* camera12, const float* model12, float max_dist, float min_dot,
+                              double* sums, int64_t* count);
 void launch_icp_begin_level(kf_context* ctx, int level_index);
 void launch_fill_voxels(kf_context* ctx);
 // skip and free bits from the observed and part bits; end_frame: the launch
