@@ -226,6 +226,9 @@ typedef struct kf_frame_result {
   double proposal_translation_error; /* m */
   double proposal_rotation_error;    /* rad */
   double learned_rms;                /* m */
+  /* KF_INIT_LEARNED: the seed tracking started from, T_previous [R(omega) | t]
+   * of the network's twist (camera_to_world, row-major FP64) */
+  double proposal_pose[16];
 } kf_frame_result;
 
 typedef struct kf_context kf_context;

@@ -96,7 +96,7 @@ class KfFrameResult(C.Structure):
     _fields_ = [("tracked", C.c_int32), ("failed_level", C.c_int32), ("used_learned", C.c_int32),
                 ("pad", C.c_int32), ("pose", C.c_double * 16), ("level", KfIcpResult * 4),
                 ("proposal_translation_error", C.c_double), ("proposal_rotation_error", C.c_double),
-                ("learned_rms", C.c_double)]
+                ("learned_rms", C.c_double), ("proposal_pose", C.c_double * 16)]
 
 
 class KfPosenetConfig(C.Structure):

@@ -919,6 +919,7 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
         result->proposal_translation_error = ctx->icp_host->proposal_err_t;
         result->proposal_rotation_error = ctx->icp_host->proposal_err_r;
         result->learned_rms = ctx->icp_host->learned_rms;
+        std::memcpy(result->proposal_pose, ctx->icp_host->proposal, sizeof(result->proposal_pose));
       }
       report_sync_timeouts(ctx);
     }

@@ -69,7 +69,8 @@ extern "C" {
 //                    z in [-0.4, 1.8]) with 6 spheres of radius U(0.05, 0.2) m
 //                    whose surfaces lie 0.5-1.2 m in front of the first camera.
 // kind 1: "hall"  -- 6 m x 3 m x 4 m room (x, y, z) with 20 boxes and 20
-//                    spheres; the camera pans and walks along x.
+//                    spheres; the camera pans (+-18 deg) and walks along x
+//                    (+-0.38 m) back and forth, one sweep per 300 frames.
 void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, double fx, double fy,
                       double cx, double cy) {
   auto* sc = new SceneDesc();
@@ -113,8 +114,9 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
       sc->r_amp[a] = (0.16 * 3.141592653589793 / 180.0) / sc->r_freq[a] * rng.uniform(0.6, 1.0);
       sc->r_phase[a] = rng.uniform(0.0, 6.283185307179586);
     }
-    sc->pan_rate = 0.0;
-    sc->walk_rate = 0.0;
+    sc->pan_amp = 0.0;
+    sc->walk_amp = 0.0;
+    sc->sweep_period = 0.0;
   } else {
     sc->room_lo = {-3.0, -1.5, -1.0};
     sc->room_hi = {3.0, 1.5, 3.0};
@@ -141,16 +143,20 @@ void* kf_scene_create(int kind, std::uint64_t seed, int width, int height, doubl
         sc->spheres[sc->n_spheres++] = s;
       }
     }
+    // per-frame translation <= (0.38 * 2 pi / 300 + 0.0015) m along x and
+    // 0.0015 m along y and z (<= 9.7 mm in all), rotation <= 18 deg * 2 pi /
+    // 300 + 3 * 0.05 deg (<= 0.47 deg)
     for (int a = 0; a < 3; ++a) {
       sc->t_freq[a] = rng.uniform(0.02, 0.04);
-      sc->t_amp[a] = 0.002 / sc->t_freq[a];
+      sc->t_amp[a] = 0.0015 / sc->t_freq[a];
       sc->t_phase[a] = rng.uniform(0.0, 6.283185307179586);
       sc->r_freq[a] = rng.uniform(0.02, 0.04);
       sc->r_amp[a] = (0.05 * 3.141592653589793 / 180.0) / sc->r_freq[a];
       sc->r_phase[a] = rng.uniform(0.0, 6.283185307179586);
     }
-    sc->pan_rate = 0.3 * 3.141592653589793 / 180.0;
-    sc->walk_rate = 0.004;
+    sc->pan_amp = 18.0 * 3.141592653589793 / 180.0;
+    sc->walk_amp = 0.38;
+    sc->sweep_period = 300.0;
   }
   return sc;
 }
@@ -167,9 +173,12 @@ void kf_scene_pose(const void* s, int frame, double* pose) {
     t[a] = sc.t_amp[a] * (std::sin(sc.t_freq[a] * k + sc.t_phase[a]) - std::sin(sc.t_phase[a]));
     e[a] = sc.r_amp[a] * (std::sin(sc.r_freq[a] * k + sc.r_phase[a]) - std::sin(sc.r_phase[a]));
   }
-  // hall sweep: pan about y, walk along x (bounded back-and-forth)
-  e[1] += sc.pan_rate * k;
-  t[0] += sc.walk_rate * k;
+  // hall sweep: pan about y and walk along x, back and forth
+  if (sc.sweep_period > 0.0) {
+    const double phase = 6.283185307179586 * k / sc.sweep_period;
+    e[1] += sc.pan_amp * std::sin(phase);
+    t[0] += sc.walk_amp * std::sin(phase);
+  }
   double R[9];
   euler_to_rot(e[0], e[1], e[2], R);
   for (int r = 0; r < 3; ++r) {

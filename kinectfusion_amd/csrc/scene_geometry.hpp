@@ -67,8 +67,9 @@ struct SceneDesc {
   // trajectory parameters
   double t_amp[3], t_freq[3], t_phase[3];
   double r_amp[3], r_freq[3], r_phase[3];
-  double pan_rate;   // radians per frame about the vertical axis (hall sweep)
-  double walk_rate;  // metres per frame along x (hall sweep)
+  double pan_amp;    // hall sweep: pan about the vertical axis, pan_amp sin(2 pi k / sweep_period) rad
+  double walk_amp;   // hall sweep: walk along x, walk_amp sin(2 pi k / sweep_period) m
+  double sweep_period;  // frames of one back-and-forth sweep (0: no sweep)
 };
 
 // Ray/sphere: smallest s > eps with |o + s d - c| = r.

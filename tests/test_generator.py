@@ -34,7 +34,7 @@ class SceneDesc(C.Structure):
                 ("n_boxes", C.c_int), ("spheres", Sphere * 64), ("boxes", Box * 64),
                 ("t_amp", C.c_double * 3), ("t_freq", C.c_double * 3), ("t_phase", C.c_double * 3),
                 ("r_amp", C.c_double * 3), ("r_freq", C.c_double * 3), ("r_phase", C.c_double * 3),
-                ("pan_rate", C.c_double), ("walk_rate", C.c_double)]
+                ("pan_amp", C.c_double), ("walk_amp", C.c_double), ("sweep_period", C.c_double)]
 
 
 def description(scene) -> SceneDesc:
