@@ -61,8 +61,10 @@ enum { KF_DAMPING_NONE = 0, KF_DAMPING_IDENTITY = 1, KF_DAMPING_DIAGONAL = 2 };
  * space everywhere; the band integrates only the 8^3 blocks that the
  * truncation band of some pixel crosses (TruncationBandWalk at a one-voxel
  * step, the host BandIntegrationSweep, tsdf_integration.hpp:444-497), and
- * voxels outside the band keep their values.  A hashed volume always
- * integrates its band (at half-block steps, block_rep.hpp:287). */
+ * voxels outside the band keep their values.  A hashed volume integrates
+ * its band (at half-block steps, block_rep.hpp:287) and requires
+ * KF_INTEGRATION_BAND, as the reference's sparse storage does
+ * (Pipeline::require_valid_storage, src/kinectfusion/pipeline.cpp:108-117). */
 enum { KF_INTEGRATION_FULL = 0, KF_INTEGRATION_BAND = 1 };
 /* How kf_process_frame seeds ICP. */
 enum { KF_INIT_PREVIOUS = 0, KF_INIT_CONSTANT_VELOCITY = 1, KF_INIT_LEARNED = 2 };
@@ -91,7 +93,7 @@ typedef struct kf_config {
   int32_t projective_distance;
   int32_t distance_scaled_truncation;
   float truncation_distance_scale;
-  int32_t integration_mode;  /* KF_INTEGRATION_*; dense storage only */
+  int32_t integration_mode;  /* KF_INTEGRATION_*; KF_INTEGRATION_BAND with hashed storage */
   /* RaycastOptions */
   float ray_min_depth;
   float ray_max_depth;

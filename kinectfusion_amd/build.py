@@ -4,6 +4,7 @@ Produces, in-tree:
   kinectfusion_amd/libkf_hip.so    -- gfx950 HIP kernels + the C ABI (include/kf_hip.h)
   kinectfusion_amd/libkf_scene.so  -- seeded synthetic RGB-D scenes (host C++ renderer and the same
                                       per-pixel renderer as a gfx950 kernel)
+  kinectfusion_amd/libkf_io.so     -- host image codec helper of the reconstruction app (PNG row filters)
 
 HIP flags: gfx950 only, -O3, no FMA contraction (bit-exact parity with the
 reference's float expression order), IEEE fp32 divide/sqrt (hip default),
@@ -74,7 +75,11 @@ def build(jobs: int = 8) -> dict[str, Path]:
     dev_obj = _hip_object("scene_device")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", str(host_obj), str(dev_obj), "-lgomp", "-o", str(tmp)])
     os.replace(tmp, scene)
-    return {"hip": lib, "scene": scene}
+    io = PKG / "libkf_io.so"
+    tmp = PKG / "libkf_io.so.tmp"
+    _run(["g++", "-std=c++20", "-O2", "-fPIC", "-shared", str(CSRC / "kf_io.cpp"), "-o", str(tmp)])
+    os.replace(tmp, io)
+    return {"hip": lib, "scene": scene, "io": io}
 
 
 if __name__ == "__main__":

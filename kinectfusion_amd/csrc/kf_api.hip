@@ -224,6 +224,10 @@ void validate(const kf_config& c) {
   require(c.tsdf_rule == KF_RULE_CLASSIC || c.tsdf_rule == KF_RULE_ANGLE_WEIGHTED, "Unknown TSDF rule");
   require(c.integration_mode == KF_INTEGRATION_FULL || c.integration_mode == KF_INTEGRATION_BAND,
           "Unknown integration mode");
+  // Pipeline::require_valid_storage (src/kinectfusion/pipeline.cpp:108-117)
+  require(c.storage != KF_STORAGE_HASHED || c.integration_mode == KF_INTEGRATION_BAND,
+          "Sparse storage implies band integration (nothing is allocated far from surfaces); set "
+          "integration_mode = KF_INTEGRATION_BAND");
   // TsdfIntegrator::validated (tsdf_integration.cpp) and depth options
   require(c.depth_scale > 0.0F, "Depth scale must be positive");
   require(c.min_depth >= 0.0F && c.max_depth > c.min_depth, "Depth range is invalid");

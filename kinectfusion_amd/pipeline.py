@@ -107,6 +107,10 @@ class PipelineConfig:
     voxel: str = "float"                       # --voxel float|quantized|bf16
     color: str = "float"                       # --color float|none
     storage: str = "dense"
+    # RaycastBackend (raycasting.hpp:22): "march" and "bitmap-march" render
+    # identical maps here (the march skips empty blocks exactly);
+    # "band-march" (approximate in the reference) is not provided
+    raycast_backend: str = "march"
     volume: VolumeGeometry = field(default_factory=VolumeGeometry)
     integration: TsdfIntegrationOptions = field(default_factory=TsdfIntegrationOptions)
     raycast: RaycastOptions = field(default_factory=RaycastOptions)
@@ -130,6 +134,15 @@ class PipelineConfig:
     def native(self) -> N.KfConfig:
         if self.space != "hip":
             raise NotImplementedError(f"memory space '{self.space}' is not built; this library provides 'hip' only")
+        if self.raycast_backend not in ("march", "bitmap-march"):
+            raise NotImplementedError(f"raycast backend '{self.raycast_backend}' is not provided; 'march' and "
+                                      "'bitmap-march' render identical maps")
+        # Pipeline::require_valid_storage (src/kinectfusion/pipeline.cpp:108-117)
+        if self.storage == "hashed" and self.integration.mode != "band":
+            raise ValueError("Sparse storage implies band integration (nothing is allocated far from surfaces); "
+                             "set integration = 'band'")
+        if self.storage == "hashed" and self.raycast_backend != "march":
+            raise ValueError("Sparse storage supports the plain march raycast backend")
         c = N.KfConfig()
         N.hip().kf_config_defaults(C.byref(c))
         c.resolution[:] = list(self.volume.resolution)

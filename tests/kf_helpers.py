@@ -17,10 +17,12 @@ def config1_geometry(res=128, voxel=0.01, margin=0.5, trunc=0.05):
 def pipeline_config(K, res=128, voxel=0.01, margin=0.5, trunc=0.05, store="float", rule="angle-weighted",
                     color="float", radius=3, sigma_s=4.5, sigma_r=0.03, levels=3, **kw):
     r, v, origin, t = config1_geometry(res, voxel, margin, trunc)
+    # sparse storage integrates its band (Pipeline::require_valid_storage)
+    mode = "band" if kw.get("storage") == "hashed" else "full"
     return K.PipelineConfig(
         voxel=store, tsdf_rule=rule, color=color,
         volume=K.VolumeGeometry((r, r, r), v, origin, t),
-        integration=K.TsdfIntegrationOptions(depth_scale=1000.0),
+        integration=K.TsdfIntegrationOptions(depth_scale=1000.0, mode=mode),
         depth=K.DepthProcessingOptions(levels=levels, bilateral_radius=radius, bilateral_spatial_sigma=sigma_s,
                                        bilateral_depth_sigma=sigma_r),
         intrinsics=K.CameraIntrinsics(*INTR), width=W, height=H, **kw)
