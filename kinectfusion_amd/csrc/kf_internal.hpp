@@ -302,8 +302,9 @@ struct IcpState {
 // the trial state is cleared and the model pose is the running pose.  In a
 // failed frame nothing runs; a frame whose learned run was kept (marked
 // failed so that the fallback pass exits at once) keeps its results.
-__device__ inline void icp_begin_level(IcpState* st, int level) {
-  st->level_sync[0] = 0U;
+// Prepare a level (one thread): result slot, status and counters cleared,
+// model pose = running pose.  The grid-wide ticket is left alone.
+__device__ inline void icp_begin_level_state(IcpState* st, int level) {
   if (st->frame_failed != 0) {
     if (st->used_learned == 0) {
       st->results[level] = kf_icp_result{};
@@ -330,6 +331,14 @@ __device__ inline void icp_begin_level(IcpState* st, int level) {
   double inv[16];
   rigid_inverse64(st->model_pose, inv);
   for (int i = 0; i < 12; ++i) st->model_inv[i] = static_cast<float>(inv[i]);
+}
+
+// The same, also resetting the grid-wide ticket (a timed-out one-launch
+// level may have left it short); runs before any workgroup of the level's
+// one-launch form can take a ticket.
+__device__ inline void icp_begin_level(IcpState* st, int level) {
+  st->level_sync[0] = 0U;
+  icp_begin_level_state(st, level);
 }
 
 // The learned warm start's check after its run (one thread): keep the run
@@ -521,6 +530,11 @@ void launch_icp_reduce_static(hipStream_t stream, const float4* lv_v, const floa
                               const float* camera12, const float* model12, float max_dist, float min_dot,
                               double* sums, int64_t* count);
 void launch_icp_begin_level(kf_context* ctx, int level_index);
+// The learned warm start's fallback pass (every level: raycast, then the
+// trials) as one launch that returns at once when the learned run was kept;
+// false when its grid cannot be resident at once on this device (the caller
+// then launches each level on its own).
+bool launch_fallback_pass(kf_context* ctx, const DeviceIcpParams& p, const int32_t* iterations);
 void launch_fill_voxels(kf_context* ctx);
 // skip and free bits from the observed and part bits; end_frame: the launch
 // also records the frame's pose (trajectory slot, previous poses, counter)

@@ -1,0 +1,204 @@
+// The raycast of one pixel (reference: SurfaceRaycast::render_pixel /
+// find_zero_crossing / write_surface_sample, include/kinectfusion/
+// raycasting.hpp:116-245), shared by the raycast kernel and the learned warm
+// start's one-launch fallback pass.  Corner blends visit the 2^3 stencil in
+// z -> y -> x order with the reference's weight products, so every sample is
+// bit-identical to the CPU sampler.
+#pragma once
+
+#include "kf_sampler.hpp"
+
+namespace kf {
+
+struct RayParams {
+  Intr k;
+  int w, h;
+  float min_depth, max_depth, step_scale;
+  int require_all;
+  int cell_normals;
+  int seed_previous;
+};
+
+__device__ inline uint32_t pixel_from_color(V3 c) {
+  auto to_byte = [](float v) { return static_cast<uint32_t>(fclamp_ref(v, 0.0F, 255.0F)); };
+  return to_byte(c.x) | (to_byte(c.y) << 8U) | (to_byte(c.z) << 16U) | (0xFFU << 24U);
+}
+
+// Pixel (x, y) of the maps (pts, nrm, col) rendered from `pose`.
+template <int Kind, typename Store>
+__device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayParams& rp, const Rigid& pose, int x,
+                                     int y, float4* __restrict__ pts, float4* __restrict__ nrm,
+                                     uint32_t* __restrict__ col, const uint32_t* __restrict__ skip_bits,
+                                     const uint32_t* __restrict__ free_bits, const BlockMap& bm) {
+  const size_t i = static_cast<size_t>(y) * rp.w + x;
+  // RaycastOptions::seed_from_previous (raycasting.hpp:122-131): start just
+  // in front of the surface this pixel found in the previous raycast into
+  // this map (a zero point is the maps' initial fill, not a surface)
+  float seed = rp.min_depth;
+  if (rp.seed_previous != 0) {
+    const V3 previous = ld3(pts[i]);
+    if (finite3(previous) && dot3(previous, previous) > 0.0F) {
+      const float previous_depth = norm3(sub3(previous, pose.t));
+      seed = fmax_ref(rp.min_depth, previous_depth - (2.0F * vol.g.trunc));
+    }
+  }
+  V3 P = nan3(), N = nan3();
+  uint32_t C = 0U;
+  const V3 translated = back_project(rp.k, static_cast<float>(x), static_cast<float>(y), 1.0F);
+  const V3 direction = rot(pose, translated);
+  const float dn = norm3(direction);
+  if (dn > 0.0F) {
+    const V3 dir = div3(direction, dn);
+    const float base_step = vol.g.voxel * rp.step_scale;
+    bool have_prev = false, hit = false;
+    float prev_t = 0.0F;
+    V3 prev_p = mk3(0.0F, 0.0F, 0.0F), surface = prev_p;
+    float len = seed;
+    // Samples whose outcome is known are stepped over without touching the
+    // volume, reproducing the reference's float sequence of ray lengths:
+    // before the ray enters the volume box, after it has left it (it cannot
+    // come back), inside 8^3 blocks whose 3x3x3 block neighbourhood holds no
+    // observed voxel (no TSDF), and inside blocks whose neighbourhood holds
+    // only observed voxels reading exactly 1 (the trilinear blend of eight
+    // corners of 1 is exactly 1: the weighted sum and the weight sum are the
+    // same float sum).  The output is identical to marching every sample.
+    const double inv = 1.0 / static_cast<double>(vol.g.voxel);
+    const double ga[3] = {(static_cast<double>(pose.t.x) - vol.g.ox) * inv,
+                          (static_cast<double>(pose.t.y) - vol.g.oy) * inv,
+                          (static_cast<double>(pose.t.z) - vol.g.oz) * inv};
+    const double gb[3] = {static_cast<double>(dir.x) * inv, static_cast<double>(dir.y) * inv,
+                          static_cast<double>(dir.z) * inv};
+    const double dims[3] = {static_cast<double>(vol.g.nx), static_cast<double>(vol.g.ny),
+                            static_cast<double>(vol.g.nz)};
+    double t_in = -1.0e300, t_out = 1.0e300, slope_in = 1.0, slope_out = 1.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (fabs(gb[k]) <= 1.0e-12) {
+        if (ga[k] < 0.0 || ga[k] > dims[k]) t_out = -1.0e300;  // parallel and outside
+        continue;
+      }
+      double t0 = -ga[k] / gb[k], t1 = (dims[k] - ga[k]) / gb[k];
+      if (t0 > t1) {
+        const double tmp = t0;
+        t0 = t1;
+        t1 = tmp;
+      }
+      if (t0 > t_in) {
+        t_in = t0;
+        slope_in = fabs(gb[k]);
+      }
+      if (t1 < t_out) {
+        t_out = t1;
+        slope_out = fabs(gb[k]);
+      }
+    }
+    // margins of 0.01 voxel along the deciding axis absorb the rounding of
+    // the float grid coordinates the reference compares
+    const float len_stop = t_out < t_in ? -1.0F : static_cast<float>(fmin(t_out + 0.01 / slope_out, 1.0e30));
+    const double len_enter = t_in - 0.01 / slope_in;
+    while (static_cast<double>(len) < len_enter && len <= rp.max_depth) len += base_step;
+    const float free_step = fmax_ref(base_step, (1.0F * vol.g.trunc) * rp.step_scale);
+    int cur_blk = -1;
+    bool cur_skip = false, cur_free = false;
+    while (len <= rp.max_depth) {
+      if (len > len_stop) break;
+      const V3 point = add3(pose.t, scale3(dir, len));
+      const V3 gc = vol.grid_coords(point);
+      if (vol.contains(gc)) {
+        // the block of this sample, classified exactly as the sampler sees it
+        const int fb = bm.flat(static_cast<int>(gc.x) >> 3, static_cast<int>(gc.y) >> 3, static_cast<int>(gc.z) >> 3);
+        if (fb != cur_blk) {
+          // both words in one round trip: the class reads both unconditionally
+          // (a block is never both skip and free); with the free word read
+          // only for a non-skip block, each new block cost two dependent
+          // reads (1.5 % of the frame rate)
+          cur_blk = fb;
+          const uint32_t sh = static_cast<uint32_t>(fb & 31);
+          const uint32_t cls = ((skip_bits[fb >> 5] >> sh) & 1U) | (((free_bits[fb >> 5] >> sh) & 1U) << 1);
+          cur_skip = cls == 1U;
+          cur_free = cls == 2U;
+        }
+        if (cur_skip) {  // no corner is observed: no TSDF
+          have_prev = false;
+          len += base_step;
+          continue;
+        }
+        if (cur_free) {  // every corner reads exactly 1: no read, one free step
+          if (have_prev && prev_t < 0.0F) break;
+          have_prev = true;
+          prev_t = 1.0F;
+          prev_p = point;
+          len += free_step;
+          continue;
+        }
+      }
+      float t;
+      if (!vol.tsdf(point, rp.require_all != 0, t)) {
+        have_prev = false;
+        len += base_step;
+        continue;
+      }
+      if (have_prev && prev_t > 0.0F && t <= 0.0F) {
+        const float gap = prev_t - t;
+        const float alpha = gap != 0.0F ? prev_t / gap : 0.0F;
+        surface = add3(prev_p, scale3(sub3(point, prev_p), alpha));
+        hit = true;
+        break;
+      }
+      if (have_prev && prev_t < 0.0F && t > 0.0F) break;
+      have_prev = true;
+      prev_t = t;
+      prev_p = point;
+      const float pre = t * vol.g.trunc * rp.step_scale;
+      const float step = fmax_ref(base_step, pre);
+      len += t > 0.0F ? step : base_step;
+    }
+    if (hit) {
+      V3 n;
+      const bool ok = rp.cell_normals ? vol.cell_normal(surface, n)
+                                      : vol.normal(surface, rp.require_all != 0, n);
+      if (ok) {
+        P = surface;
+        N = n;
+        if (vol.s.has_colors()) {
+          V3 c;
+          if (vol.color(surface, c)) C = pixel_from_color(c);
+        } else {
+          const V3 view = normalize3(sub3(pose.t, surface));
+          const float inten = 255.0F * fmax_ref(0.0F, dot3(n, view));
+          C = pixel_from_color(mk3(inten, inten, inten));
+        }
+      }
+    }
+  }
+  pts[i] = st3(P);
+  nrm[i] = st3(N);
+  col[i] = C;
+}
+
+// Pixel of thread t (0..255) of 16 x 16 tile `tile` in a grid `tiles_x`
+// tiles wide: each wave takes an 8 x 8 quadrant (neighbouring rays march
+// alike, so a squarer wave diverges less than a 16 x 4 strip).
+__device__ inline void tile_pixel(int tile, int tiles_x, int t, int& x, int& y) {
+  x = (tile % tiles_x) * 16 + ((t >> 6) & 1) * 8 + (t & 7);
+  y = (tile / tiles_x) * 16 + (t >> 7) * 8 + ((t >> 3) & 7);
+}
+
+// The raycast options of the context's level `level` (its camera and size).
+inline RayParams ray_params(const kf_context* ctx, int level) {
+  const kf_config& c = ctx->cfg;
+  const LevelBuffers& L = ctx->lv[level];
+  RayParams rp{};
+  rp.k = L.k;
+  rp.w = L.w;
+  rp.h = L.h;
+  rp.min_depth = c.ray_min_depth;
+  rp.max_depth = c.ray_max_depth;
+  rp.step_scale = c.step_scale;
+  rp.require_all = c.require_all_corners;
+  rp.cell_normals = c.cell_normals;
+  rp.seed_previous = c.seed_from_previous;
+  return rp;
+}
+
+}  // namespace kf
