@@ -20,6 +20,13 @@ Pipelines timed on the same frames:
 The pose network is trained first (untimed) on pairs the pipeline generates
 from 64 rooms other than the evaluated one (room seeds 1-64; the timed
 trajectory is room 0; the held-out error is measured on room 1001).
+
+Beside the headline line's `roofline` (the bf16 voxel-update kernel) the JSON
+carries `roofline_float_store` (the plain FP32 pipeline's float voxel-update
+kernel), `roofline_raycast` (the headline's level-0 raycast, its samples and
+hits counted by the CPU oracle over the GPU's own volume; with the CPU leg)
+and `hashed` (the 2048^3 hashed-hall workload: frames/s, ATE and the block
+kernel's roofline).
 """
 from __future__ import annotations
 
@@ -89,11 +96,13 @@ def run_pipeline(K, torch, cfg, depth_dev, color_dev, warmup: int, steps: int, n
     return dt, traj
 
 
-def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None):
+def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None, keep: bool = False):
     """Untimed eager replays of the same frames: one with per-stage HIP
     events on the library stream (the voxel-update kernel as it runs in the
     timed loop), one counting the fused voxels of every frame (the kernel's
-    counting variant; identical poses and volumes, so identical counts)."""
+    counting variant; identical poses and volumes, so identical counts).
+    keep: the counting pipeline is returned open (its volume after the last
+    frame), with its trajectory."""
     cfg.stage_timing, cfg.count_fused = True, False
     p = K.Pipeline(cfg)
     if net is not None:
@@ -118,9 +127,24 @@ def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None):
         if i >= first:
             fused.append((now[0] - before[0], now[1] - before[1]))
         before = now
-    p.close()
     cfg.count_fused = False
+    if keep:
+        return np.array(times), np.array(fused, np.float64), learned, p
+    p.close()
     return np.array(times), np.array(fused, np.float64), learned
+
+
+def integration_roofline(times, fused, sv: int, kernel: str) -> dict:
+    """Algorithmic bytes of the voxel-update kernel per launch (read and
+    write of every fused voxel and colour voxel: N_fused 2 s_v + N_colour 2
+    16) over its mean launch time from HIP events on the library stream."""
+    alg = fused[:, 0] * 2 * sv + fused[:, 1] * 2 * 16
+    kern_ms = times[:, 5]
+    achieved = float(alg.sum() / (kern_ms.sum() / 1e3) / 1e9)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "kernel": kernel, "bytes_per_launch": float(alg.mean()),
+            "avg_launch_ms": float(kern_ms.mean()), "fused_voxels_per_launch": float(fused[:, 0].mean()),
+            "colour_voxels_per_launch": float(fused[:, 1].mean())}
 
 
 def train_posenet(K, cfg, pairs_n: int, epochs: int):
@@ -184,6 +208,71 @@ def cpu_baseline(scene, frames: int):
     return timed / t_total, t_total, timed, np.stack(poses)
 
 
+def raycast_algorithmic(p, pose, voxel: str, raycast_ms: float) -> dict:
+    """Algorithmic bytes of one 640x480 raycast of the pipeline's volume at
+    `pose`, with the samples and hits counted by the CPU oracle's raycast of
+    the same voxels: 28 B written per pixel (point, normal, colour), 8
+    corners x s_v read per sample, and per hit the normal's 6 samples (48
+    corners x s_v) and the colour's 8 colour voxels (16 B each); over the
+    level-0 raycast's time from HIP events."""
+    from oracle import kf_oracle as ko
+
+    vox, col = p.host_view()
+    half = np.float32(0.5) * np.float32(RES) * np.float32(VOXEL)
+    kind = {"float": ko.FLOAT, "quantized": ko.QUANTIZED, "bf16": ko.BF16}[voxel]
+    vol = ko.Volume.create(RES, VOXEL, (-float(half), -float(half), -MARGIN), TRUNC, kind, col is not None)
+    vol.voxels[:] = vox
+    if col is not None:
+        vol.colors[:] = col
+    del vox, col
+    pts, _, _, samples = ko.raycast(vol, pose, INTR, 640, 480)
+    hits = int(np.isfinite(pts).all(-1).sum())
+    sv = 8 if voxel == "float" else 4
+    npx = 640 * 480
+    alg = 28 * npx + samples * 8 * sv + hits * (48 * sv + 8 * 16)
+    achieved = alg / (raycast_ms / 1e3) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "kernel": f"raycast_kernel<{voxel}>, 640x480",
+            "bytes_per_launch": float(alg), "avg_launch_ms": raycast_ms, "samples": int(samples), "hits": hits,
+            "samples_per_ray": round(samples / npx, 2)}
+
+
+def hashed_bench(K, torch, steps: int, warmup: int) -> dict:
+    """The sparse hashed volume workload: 2048^3 voxels of 5 mm in 8^3
+    blocks, 2^20 hash buckets, a 500 000-block pool, float voxels without
+    colour, the seeded 6 x 3 x 4 m hall with 40 boxes and spheres, plain
+    previous-pose tracking (10/5/4).  Frames rendered on the device."""
+    scene = K.SyntheticScene("hall", seed=0)
+    n = warmup + steps
+    gt = np.stack([scene.pose(i) for i in range(n)])
+    depth = scene.render_device(gt, np.arange(n))
+    frames = [depth[i] for i in range(n)]
+    half = np.float32(0.5) * np.float32(2048) * np.float32(VOXEL)
+    cfg = K.PipelineConfig(
+        voxel="float", init="previous", tsdf_rule="angle-weighted", color="none", storage="hashed",
+        hash_buckets=1 << 20, block_capacity=500_000,
+        volume=K.VolumeGeometry((2048, 2048, 2048), VOXEL, (-float(half), -float(half), -MARGIN), TRUNC),
+        integration=K.TsdfIntegrationOptions(depth_scale=1000.0, mode="band"),
+        depth=K.DepthProcessingOptions(levels=3, bilateral_radius=3, bilateral_spatial_sigma=4.5,
+                                       bilateral_depth_sigma=0.03),
+        intrinsics=K.CameraIntrinsics(*INTR), width=640, height=480, icp_iterations=(10, 5, 4))
+    dt, traj = run_pipeline(K, torch, cfg, frames, [None] * n, warmup, steps)
+    times, fused, _ = stage_pass(K, cfg, frames, [None] * n, warmup, min(steps, 40))
+    p = K.Pipeline(cfg)
+    for i in range(n):
+        p.process_frame(frames[i], None, report=False, sync_inputs=False)
+    stats = p.hashed_stats()
+    p.close()
+    roof = integration_roofline(times, fused, 8, "integrate_blocks_kernel<float>")
+    return {"workload": "2048^3-5mm-hashed-hall-640x480", "frames": n, "fps": round(steps / dt, 2),
+            "ate_m": round(ate(traj, gt[: len(traj)]), 6), "blocks_allocated": stats["allocated"],
+            "overflow": stats["overflow"],
+            "stage_ms": {k: round(float(np.mean(times[:, j])), 4) for j, k in
+                         enumerate(["pyramid", "proposal", "raycast", "icp", "integrate", "integrate_kernel",
+                                    "total"])},
+            "roofline": roof}
+
+
 def pose_gap(a: np.ndarray, b: np.ndarray):
     """Largest rotation angle (rad) and translation distance (m) between
     matching camera_to_world poses of two trajectories."""
@@ -230,6 +319,8 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=12)
     ap.add_argument("--only", choices=["headline", "fp32", "bf16_plain"], default=None,
                     help="time one pipeline only (profiling)")
+    ap.add_argument("--no-hashed", action="store_true", help="skip the 2048^3 hashed-volume line")
+    ap.add_argument("--hashed-steps", type=int, default=90)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run one plain device copy of this many MiB (the reference dispatch for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -287,19 +378,38 @@ def main() -> int:
     voxel, init = runs[head]
     log("stage passes: HIP events on the library stream, then exact fused-voxel counts")
     n_stage = min(args.steps, 40)
-    times, fused, learned_used = stage_pass(K, make_config(K, voxel, init), depth_dev, color_dev, args.warmup,
-                                            n_stage, net if init == "learned" else None)
+    times, fused, learned_used, kept = stage_pass(K, make_config(K, voxel, init), depth_dev, color_dev,
+                                                  args.warmup, n_stage, net if init == "learned" else None,
+                                                  keep=True)
     sv = 4 if voxel == "bf16" else 8
-    alg_bytes = fused[:, 0] * 2 * sv + fused[:, 1] * 2 * 16  # read + write of each fused voxel (+ colour)
-    kern_ms = times[:, 5]
-    achieved = float(alg_bytes.sum() / (kern_ms.sum() / 1e3) / 1e9)
+    roof = integration_roofline(times, fused, sv, f"integrate_kernel<{voxel}, colour>")
     stage_ms = {k: round(float(np.mean(times[:, j])), 4) for j, k in
                 enumerate(["pyramid", "proposal", "raycast", "icp", "integrate", "integrate_kernel", "total"])}
+    roof_float = None
+    if "fp32" in results and head != "fp32":
+        log("stage passes of the plain FP32 pipeline (float-store roofline)")
+        t32, f32, _ = stage_pass(K, make_config(K, "float", "previous"), depth_dev, color_dev, args.warmup, n_stage)
+        roof_float = integration_roofline(t32, f32, 8, "integrate_kernel<float, colour>")
 
     traffic, traffic_src = committed_traffic(voxel)
+    roof["traffic"] = traffic
+    roof["traffic_source"] = traffic_src
+
+    hashed = None
+    if not args.no_hashed:
+        log(f"hashed volume: 2048^3 hall, {args.hashed_steps} timed frames")
+        hashed = hashed_bench(K, torch, args.hashed_steps, args.warmup)
+        log(f"hashed: {hashed['fps']} frames/s, ATE {hashed['ate_m'] * 1000:.3f} mm")
 
     cpu = None
+    raycast_roof = None
     if not args.no_cpu:
+        # the level-0 raycast of the headline's volume at its last pose, its
+        # samples counted by the oracle (the learned pass raycasts level 0 only,
+        # so the stage's raycast time is that launch's)
+        if init == "learned":
+            log("raycast samples of the headline volume counted by the CPU oracle")
+            raycast_roof = raycast_algorithmic(kept, kept.trajectory()[-1], voxel, stage_ms["raycast"])
         log(f"cpu baseline: oracle on {args.cpu_frames} tracked frames (single thread)")
         fps_cpu, secs, n, oracle_poses = cpu_baseline(scene, args.cpu_frames)
         cpu = {"value": fps_cpu, "unit": "frames/s", "cores": 1, "kind": "port",
@@ -332,13 +442,15 @@ def main() -> int:
                    "learned_iterations": list(LEARNED_ITERATIONS) if init == "learned" else None, "color": "float"},
         "ate_m": round(r["ate_m"], 6),
         "stage_ms": stage_ms,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": f"integrate_kernel<{voxel}, colour>",
-                     "bytes_per_launch": float(alg_bytes.mean()), "avg_launch_ms": float(kern_ms.mean()),
-                     "traffic_source": traffic_src},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if roof_float is not None:
+        line["roofline_float_store"] = roof_float
+    if raycast_roof is not None:
+        line["roofline_raycast"] = raycast_roof
+    if hashed is not None:
+        line["hashed"] = hashed
     if init == "learned":
         line["learned_seed_kept"] = f"{learned_used}/{n_stage}"
         line["posenet"] = net_info
@@ -350,6 +462,7 @@ def main() -> int:
         line["bf16_plain_fps"] = round(results["bf16_plain"]["fps"], 2)
         line["bf16_plain_ate_m"] = round(results["bf16_plain"]["ate_m"], 6)
     print(json.dumps(line))
+    kept.close()
     if net is not None:
         net.close()
     return 0
