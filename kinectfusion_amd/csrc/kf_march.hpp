@@ -24,6 +24,22 @@ __device__ inline uint32_t pixel_from_color(V3 c) {
   return to_byte(c.x) | (to_byte(c.y) << 8U) | (to_byte(c.z) << 16U) | (0xFFU << 24U);
 }
 
+// One axis of a block's safe interior: block b covers grid coordinates
+// [8b, min(8b + 8, n)); the ray's coordinate o + d * len lies at least
+// `safe` inside it for len in [enter, exit), intersected into the running
+// interval.  A ray (almost) parallel to the axis keeps its coordinate.
+__device__ inline void block_span(int b, float o, float d, int n, float safe, float& enter, float& exit) {
+  const float lo = static_cast<float>(b * 8) + safe, hi = static_cast<float>(min(b * 8 + 8, n)) - safe;
+  if (fabsf(d) <= 1.0e-12F) {
+    if (o < lo || o > hi) exit = -3.0e38F;
+    return;
+  }
+  const float r = __builtin_amdgcn_rcpf(d);  // 1 ulp: far inside the margin
+  const float t0 = (lo - o) * r, t1 = (hi - o) * r;
+  enter = fmaxf(enter, fminf(t0, t1));
+  exit = fminf(exit, fmaxf(t0, t1));
+}
+
 // Pixel (x, y) of the maps (pts, nrm, col) rendered from `pose`.
 template <int Kind, typename Store>
 __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayParams& rp, const Rigid& pose, int x,
@@ -100,13 +116,25 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
     const float free_step = fmax_ref(base_step, (1.0F * vol.g.trunc) * rp.step_scale);
     int cur_blk = -1;
     bool cur_skip = false, cur_free = false;
+    // The samples of a skip or free block that lie deeper than kSafe voxels
+    // inside it on every axis are stepped in a tight loop (a float add and a
+    // compare each): [enter, exit) is that stretch of ray lengths, from the
+    // ray in grid coordinates, go + gd * len.  The grid coordinates the
+    // march computes for a sample differ from the exact ones by a few float
+    // ulps of the magnitudes involved (< 4e-3 voxel for points within 2^14
+    // voxels of the grid origin), and the interval's own rounding adds as
+    // little, so every sample in it falls in the block as the per-sample
+    // path would classify it.
+    constexpr float kSafe = 0.05F;
+    float enter = 0.0F, exit = -1.0F;
     while (len <= rp.max_depth) {
       if (len > len_stop) break;
       const V3 point = add3(pose.t, scale3(dir, len));
       const V3 gc = vol.grid_coords(point);
       if (vol.contains(gc)) {
         // the block of this sample, classified exactly as the sampler sees it
-        const int fb = bm.flat(static_cast<int>(gc.x) >> 3, static_cast<int>(gc.y) >> 3, static_cast<int>(gc.z) >> 3);
+        const int gx = static_cast<int>(gc.x) >> 3, gy = static_cast<int>(gc.y) >> 3, gz = static_cast<int>(gc.z) >> 3;
+        const int fb = bm.flat(gx, gy, gz);
         if (fb != cur_blk) {
           // both words in one round trip: the class reads both unconditionally
           // (a block is never both skip and free); with the free word read
@@ -117,18 +145,34 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
           const uint32_t cls = ((skip_bits[fb >> 5] >> sh) & 1U) | (((free_bits[fb >> 5] >> sh) & 1U) << 1);
           cur_skip = cls == 1U;
           cur_free = cls == 2U;
+          if (cls != 0U) {
+            const V3 go = vol.grid_coords(pose.t), gd = scale3(dir, vol.inv_voxel);
+            enter = -3.0e38F;
+            exit = 3.0e38F;
+            block_span(gx, go.x, gd.x, vol.g.nx, kSafe, enter, exit);
+            block_span(gy, go.y, gd.y, vol.g.ny, kSafe, enter, exit);
+            block_span(gz, go.z, gd.z, vol.g.nz, kSafe, enter, exit);
+          }
         }
         if (cur_skip) {  // no corner is observed: no TSDF
           have_prev = false;
           len += base_step;
+          if (len >= enter)
+            while (len < exit && len <= rp.max_depth) len += base_step;
           continue;
         }
-        if (cur_free) {  // every corner reads exactly 1: no read, one free step
+        if (cur_free) {  // every corner reads exactly 1: no read, free steps
           if (have_prev && prev_t < 0.0F) break;
           have_prev = true;
           prev_t = 1.0F;
-          prev_p = point;
+          float last = len;
           len += free_step;
+          if (len >= enter)
+            while (len < exit && len <= rp.max_depth) {
+              last = len;
+              len += free_step;
+            }
+          prev_p = add3(pose.t, scale3(dir, last));  // the last free sample's point
           continue;
         }
       }
