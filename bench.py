@@ -11,9 +11,10 @@ tracking (raycast + ICP per level), integration.  Frames are resident in HBM
 before the timed region.
 
 Pipelines timed on the same frames:
-  * headline: BF16 voxels + learned warm start (pose network proposal, 4
+  * headline: BF16 voxels + learned warm start (pose network proposal, 2
     level-0 ICP iterations, fallback to the constant-velocity seed with the
-    full 10/5/4 schedule when the final RMS exceeds 1 cm);
+    full 10/5/4 schedule when the final RMS exceeds 1 cm or the last
+    increment 1 mm / 1 mrad);
   * plain FP32: float voxels, previous-pose seed, 10/5/4 iterations
     (the reference's method);
   * BF16 plain: bf16 voxels with the plain method.
@@ -96,13 +97,15 @@ def run_pipeline(K, torch, cfg, depth_dev, color_dev, warmup: int, steps: int, n
     return dt, traj
 
 
-def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None, keep: bool = False):
+def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None, keep: bool = False,
+               increments: list | None = None):
     """Untimed eager replays of the same frames: one with per-stage HIP
     events on the library stream (the voxel-update kernel as it runs in the
     timed loop), one counting the fused voxels of every frame (the kernel's
     counting variant; identical poses and volumes, so identical counts).
     keep: the counting pipeline is returned open (its volume after the last
-    frame), with its trajectory."""
+    frame), with its trajectory.  increments: receives (translation,
+    rotation, status) of each kept learned run's finest level."""
     cfg.stage_timing, cfg.count_fused = True, False
     p = K.Pipeline(cfg)
     if net is not None:
@@ -115,6 +118,9 @@ def stage_pass(K, cfg, depth_dev, color_dev, first: int, count: int, net=None, k
             times.append([st.pyramid_ms, st.proposal_ms, st.raycast_ms, st.icp_ms, st.integrate_ms,
                           st.integrate_kernel_ms, st.total_ms])
             learned += res.used_learned
+            if res.used_learned and increments is not None:
+                r0 = res.level[0]
+                increments.append((r0.update_translation, r0.update_rotation, r0.status))
     p.close()
     cfg.stage_timing, cfg.count_fused = False, True
     p = K.Pipeline(cfg)
@@ -378,11 +384,16 @@ def main() -> int:
     voxel, init = runs[head]
     log("stage passes: HIP events on the library stream, then exact fused-voxel counts")
     n_stage = min(args.steps, 40)
+    increments: list = []
     times, fused, learned_used, kept = stage_pass(K, make_config(K, voxel, init), depth_dev, color_dev,
                                                   args.warmup, n_stage, net if init == "learned" else None,
-                                                  keep=True)
+                                                  keep=True, increments=increments)
     sv = 4 if voxel == "bf16" else 8
     roof = integration_roofline(times, fused, sv, f"integrate_kernel<{voxel}, colour>")
+    inc = np.array(increments or [(0.0, 0.0, 0)], np.float64)
+    learned_increment = {"kept_runs": len(increments), "converged": int((inc[:, 2] == 1).sum()),
+                         "translation_m_p50_p90_max": [float(np.percentile(inc[:, 0], q)) for q in (50, 90, 100)],
+                         "rotation_rad_p50_p90_max": [float(np.percentile(inc[:, 1], q)) for q in (50, 90, 100)]}
     stage_ms = {k: round(float(np.mean(times[:, j])), 4) for j, k in
                 enumerate(["pyramid", "proposal", "raycast", "icp", "integrate", "integrate_kernel", "total"])}
     roof_float = None
@@ -453,6 +464,7 @@ def main() -> int:
         line["hashed"] = hashed
     if init == "learned":
         line["learned_seed_kept"] = f"{learned_used}/{n_stage}"
+        line["learned_last_increment"] = learned_increment
         line["posenet"] = net_info
     if "fp32" in results:
         line["fp32_plain_fps"] = round(results["fp32"]["fps"], 2)

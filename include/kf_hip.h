@@ -131,11 +131,16 @@ typedef struct kf_config {
   int32_t count_fused;       /* kf_process_frame accumulates fused-voxel counts */
   /* KF_INIT_LEARNED (no reference counterpart): per-level iterations run from
    * the network's proposal (0 skips the level and its raycast), and the
-   * largest final point-to-plane RMS (m) at which the proposal's result is
-   * kept; otherwise the frame is tracked again from the constant-velocity
-   * seed with icp_iterations. */
+   * thresholds that keep the proposal's result: the finest level's final
+   * point-to-plane RMS (m), and its last increment (m, rad) -- a run still
+   * moving more than that at its last iteration needed more iterations than
+   * its schedule gives (0 disables the increment test).  Otherwise the
+   * frame is tracked again from the constant-velocity seed with
+   * icp_iterations. */
   int32_t learned_iterations[4];
   float learned_max_rms;
+  float learned_max_update_translation;
+  float learned_max_update_rotation;
 } kf_config;
 
 /* ProjectiveIcpOptions (icp_optimizer.hpp:82-102). */

@@ -45,6 +45,7 @@ class KfConfig(C.Structure):
         ("hash_buckets", C.c_int64), ("block_capacity", C.c_int64),
         ("init_mode", C.c_int32), ("icp_iterations", C.c_int32 * 4), ("stage_timing", C.c_int32),
         ("count_fused", C.c_int32), ("learned_iterations", C.c_int32 * 4), ("learned_max_rms", C.c_float),
+        ("learned_max_update_translation", C.c_float), ("learned_max_update_rotation", C.c_float),
     ]
 
 

@@ -266,6 +266,8 @@ void validate(const kf_config& c) {
     }
     require(any > 0, "Learned-seed iterations must run at least one level");
     require(c.learned_max_rms > 0.0F, "Learned-seed RMS threshold must be positive");
+    require(c.learned_max_update_translation >= 0.0F && c.learned_max_update_rotation >= 0.0F,
+            "Learned-seed increment thresholds must be >= 0");
   }
 }
 
@@ -408,7 +410,9 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     posenet_propose(ctx->posenet, ctx->stream, ctx->prev_coarse, ctx->lv[c.levels - 1].depth, ctx->twist);
     hipLaunchKernelGGL(learned_seed_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp, ctx->twist);
     record(ctx, 14);
-    const DeviceIcpParams p = to_device_params(opt);
+    DeviceIcpParams p = to_device_params(opt);
+    p.learned_max_t = c.learned_max_update_translation;
+    p.learned_max_r = c.learned_max_update_rotation;
     track_levels(ctx, p, timing, c.learned_iterations, false, c.learned_max_rms);  // + the check
     // the fallback pass runs only when the check failed: one launch for every
     // level, or one per level when its grid cannot be resident at once
@@ -491,6 +495,8 @@ void kf_config_defaults(kf_config* c) {
   c->learned_iterations[2] = 0;
   c->learned_iterations[3] = 0;
   c->learned_max_rms = 0.01F;
+  c->learned_max_update_translation = 1.0e-3F;
+  c->learned_max_update_rotation = 1.0e-3F;
 }
 
 void kf_icp_options_defaults(kf_icp_options* o) {

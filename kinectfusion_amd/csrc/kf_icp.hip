@@ -788,7 +788,7 @@ __device__ void finish_level_block(IcpState* st, const DeviceIcpParams& p, int l
       }
     }
   }
-  if (check_rms > 0.0F && threadIdx.x == 0) learned_check(st, level, check_rms);
+  if (check_rms > 0.0F && threadIdx.x == 0) learned_check(st, level, check_rms, p.learned_max_t, p.learned_max_r);
 }
 
 // A level with no trial: its finish alone.

@@ -342,15 +342,20 @@ __device__ inline void icp_begin_level(IcpState* st, int level) {
 }
 
 // The learned warm start's check after its run (one thread): keep the run
-// when no level failed and the finest level's point-to-plane RMS is at most
-// max_rms, marking the frame failed so that the fallback pass exits at once
+// when no level failed, the finest level's point-to-plane RMS is at most
+// max_rms and it converged or its last increment is within (max_t, max_r),
+// marking the frame failed so that the fallback pass exits at once
 // (the frame's end clears the mark); otherwise restore the constant-velocity
 // seed for the fallback pass.
-__device__ inline void learned_check(IcpState* st, int finest, float max_rms) {
+__device__ inline void learned_check(IcpState* st, int finest, float max_rms, float max_t, float max_r) {
   const kf_icp_result& r = st->results[finest];
   const double rms = r.mean_squared_residual >= 0.0 ? sqrt(r.mean_squared_residual) : 1e300;
   st->learned_rms = rms;
-  if (st->frame_failed == 0 && (r.status == 0 || r.status == 1) && rms <= static_cast<double>(max_rms)) {
+  // a run that ended at its iteration budget still moving more than the
+  // increment thresholds needed more iterations than its schedule gives
+  const bool settled = r.status == 1 || ((max_t <= 0.0F || r.update_translation <= static_cast<double>(max_t)) &&
+                                         (max_r <= 0.0F || r.update_rotation <= static_cast<double>(max_r)));
+  if (st->frame_failed == 0 && (r.status == 0 || r.status == 1) && settled && rms <= static_cast<double>(max_rms)) {
     st->used_learned = 1;
     st->frame_failed = 1;
   } else {
@@ -392,6 +397,9 @@ struct DeviceIcpParams {
   float lambda0;
   int adaptive;
   float inc, dec, lmin, lmax;
+  // the learned warm start's check: largest last increment of a kept run
+  // (0: no increment test)
+  float learned_max_t = 0.0F, learned_max_r = 0.0F;
 };
 
 struct LevelBuffers {

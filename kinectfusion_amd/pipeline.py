@@ -126,10 +126,13 @@ class PipelineConfig:
     hash_buckets: int = 1 << 20
     block_capacity: int = 500_000              # 0: a quarter of the grid's blocks (pipeline.hpp:40)
     # learned warm start: iterations per level from the network's proposal
-    # (0 skips a level) and the final RMS (m) above which the frame falls
-    # back to the constant-velocity seed with icp_iterations
+    # (0 skips a level); the frame falls back to the constant-velocity seed
+    # with icp_iterations when the finest level's final RMS (m) or its last
+    # increment (m, rad: the run needed more iterations) exceeds these
     learned_iterations: tuple[int, ...] = (4, 0, 0)
     learned_max_rms: float = 0.01
+    learned_max_update_translation: float = 1.0e-3
+    learned_max_update_rotation: float = 1.0e-3
 
     def native(self) -> N.KfConfig:
         if self.space != "hip":
@@ -182,6 +185,8 @@ class PipelineConfig:
         lits = list(self.learned_iterations) + [0] * (4 - len(self.learned_iterations))
         c.learned_iterations[:] = lits[:4]
         c.learned_max_rms = self.learned_max_rms
+        c.learned_max_update_translation = self.learned_max_update_translation
+        c.learned_max_update_rotation = self.learned_max_update_rotation
         return c
 
 

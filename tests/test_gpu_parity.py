@@ -53,8 +53,10 @@ def test_pyramid_parity(K, ko, scene, torch_gpu, levels):
     l0 = src.level(0)
     gpu_f = host(l0["depth"]).view(np.uint16)
     diff = np.abs(gpu_f.astype(np.int32) - ref[0]["depth"].astype(np.int32))
+    # the FP64 table-driven exponential rounded to float reproduces the CPU's
+    # float exponential on every weight of these frames: exact
     print(f"bilateral: {int((diff > 0).sum())} of {diff.size} pixels differ, max {diff.max()}")
-    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert diff.max() == 0
     prev = gpu_f
     for level in range(levels):
         lv = src.level(level)
@@ -83,7 +85,7 @@ def test_bilateral_other_radii(K, ko, scene, torch_gpu, radius):
     gpu_f = host(src.level(0)["depth"]).view(np.uint16)
     diff = np.abs(gpu_f.astype(np.int32) - ref[0]["depth"].astype(np.int32))
     print(f"bilateral r={radius}: {int((diff > 0).sum())} of {diff.size} pixels differ, max {diff.max()}")
-    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert diff.max() == 0
     p.close()
 
 

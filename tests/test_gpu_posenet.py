@@ -175,6 +175,59 @@ def test_learned_warm_start_tracks_and_gains(K, PN, torch_gpu):
     assert np.median(prop_err) < np.median(motion)
     assert max(gt_err) < 0.01
     p.close()
+    # what the network gains: the same reduced schedule without it, from the
+    # previous pose (the reference's seed) and from the constant-velocity seed
+    lost = {}
+    for init in ("previous", "constant-velocity"):
+        q = K.Pipeline(pipeline_config(K, voxel=0.02, init=init, icp_iterations=(4, 0, 0)))
+        tracked = 0
+        for i in range(n):
+            depth, rgba, _ = scene.frame(i * stride)
+            tracked += q.process_frame(to_dev(torch_gpu, depth), to_dev(torch_gpu, rgba)).tracked
+        tq = q.trajectory()
+        lost[init] = (max(np.linalg.norm(tq[i][:3, 3] - scene.pose(i * stride)[:3, 3]) for i in range(n)), tracked)
+        q.close()
+    print(f"without the network, (4, 0, 0) iterations: max error vs truth / tracked frames {lost}")
+    assert max(gt_err) < lost["previous"][0]
+    net.close()
+
+
+def test_learned_run_still_moving_falls_back(K, PN, torch_gpu):
+    """The learned check's iteration criterion: a run that ends at its
+    iteration budget with a last increment above learned_max_update_* needed
+    more iterations than its schedule gives, so the frame falls back to
+    constant-velocity tracking, whose trajectory it then equals bit for bit.
+    With the increment test disabled the same one-iteration runs are kept on
+    their RMS alone; every run kept under finite thresholds lies within
+    them."""
+    net = PN.PoseNet(PN.PoseNetConfig(seed=3))
+    scene = K.SyntheticScene("room", seed=0)
+    frames = [scene.frame(i * 2) for i in range(6)]
+
+    def run(init, **kw):
+        p = K.Pipeline(pipeline_config(K, voxel=0.02, init=init, learned_iterations=(1, 0, 0), **kw))
+        if init == "learned":
+            p.attach_posenet(net)
+        res = [p.process_frame(to_dev(torch_gpu, d), to_dev(torch_gpu, c)) for d, c, _ in frames]
+        traj = p.trajectory()
+        p.close()
+        return res, traj
+
+    strict, t_strict = run("learned", learned_max_update_translation=1e-9, learned_max_update_rotation=1e-9)
+    _, t_cv = run("constant-velocity")
+    assert all(r.tracked == 1 and r.used_learned == 0 for r in strict[1:])
+    assert np.array_equal(t_strict, t_cv)
+    loose, _ = run("learned", learned_max_update_translation=0.0, learned_max_update_rotation=0.0)
+    kept = [r.level[0] for r in loose[1:] if r.used_learned]
+    print(f"kept on RMS alone: {len(kept)}/{len(loose) - 1}; last increments "
+          f"{[(round(k.update_translation * 1e3, 3), round(k.update_rotation, 5)) for k in kept]} (mm, rad)")
+    assert kept and all(k.status == 0 and k.trials == 1 for k in kept)
+    bar = float(np.median([k.update_translation for k in kept]))
+    mid, _ = run("learned", learned_max_update_translation=bar, learned_max_update_rotation=1.0)
+    for r in mid[1:]:
+        if r.used_learned:
+            assert r.level[0].update_translation <= np.float32(bar)
+    print(f"kept under a bar of {bar * 1e3:.3f} mm: {sum(r.used_learned for r in mid[1:])}/{len(mid) - 1}")
     net.close()
 
 
