@@ -3,8 +3,8 @@
 Bars: integer and index work bit-exact; float TSDF, raycast maps and
 vertex/normal maps bit-exact given identical inputs (|delta| <= 1e-5 is the
 stated floor; these kernels meet it with zero difference); bilateral filter
-within one raw unit on a small fraction of pixels (the device exponential is
-an FP64 one within 4e-16 rounded to float, the reference's is glibc expf); ICP sums within
+bit-exact on the test frames (the device exponential is an FP64 one within
+4e-16 rounded to float, the reference's is glibc expf); ICP sums within
 1e-5 relative of the FP64 oracle; poses within 1e-4 rad / 1e-4 m.
 """
 import numpy as np
