@@ -158,8 +158,8 @@ __device__ void rodrigues64(const double w[3], double R[9]) {
 // for the fallback, then pose = prev_pose [R(omega) | t].
 __global__ void learned_seed_kernel(IcpState* st, const float* twist) {
   if (threadIdx.x != 0) return;
-  // the fallback pass's grid-wide ticket starts the frame at zero (a
-  // timed-out launch may have left it short)
+  // the grid-wide ticket of the fallback's one-launch levels starts the
+  // frame at zero (a timed-out launch may have left it short)
   st->level_sync[0] = 0U;
   frame_begin(st, KF_INIT_CONSTANT_VELOCITY);
   const double w[3] = {twist[0], twist[1], twist[2]};
@@ -414,9 +414,10 @@ void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color
     p.learned_max_t = c.learned_max_update_translation;
     p.learned_max_r = c.learned_max_update_rotation;
     track_levels(ctx, p, timing, c.learned_iterations, false, c.learned_max_rms);  // + the check
-    // the fallback pass runs only when the check failed: one launch for every
-    // level, or one per level when its grid cannot be resident at once
-    if (!launch_fallback_pass(ctx, p, c.icp_iterations)) track_levels(ctx, p, false, c.icp_iterations, true);
+    // the fallback runs only when the check failed (every launch of a kept
+    // frame returns at once): per level, its raycast, then one launch that
+    // runs the level's trials in step
+    track_levels(ctx, p, false, c.icp_iterations, true);
     record(ctx, 2 * c.levels + 1);  // the fallback run counts as tracking time
   } else {
     hipLaunchKernelGGL(frame_begin_kernel, dim3(1), dim3(64), 0, ctx->stream, ctx->icp,

@@ -28,7 +28,7 @@
 // at once runs each level in one launch instead (icp_level_kernel).
 #include <mutex>
 
-#include "kf_march.hpp"
+#include "kf_internal.hpp"
 
 // Partial rows of the allocation-free reduction (kf_icp_reduce_static): one
 // row of 32 floats per sweep workgroup, at most 256 workgroups (32 KiB).
@@ -797,135 +797,6 @@ __global__ void __launch_bounds__(64) icp_finish_level_kernel(IcpState* st, Devi
   finish_level_block(st, p, level, check_rms);
 }
 
-// One step of the one-launch fallback pass's grid-wide protocol: every
-// workgroup takes a ticket; the last one runs `last` (a whole-workgroup
-// job), resets the ticket and publishes the step by advancing the epoch to
-// `target` after a device-scope release; the others wait for the epoch with
-// a device-scope acquire, each wait bounded by timeout_ticks of the 100 MHz
-// wall clock.  false: the wait timed out (the frame is marked failed and the
-// event counted; the workgroup must exit).
-template <typename F>
-__device__ bool grid_step(IcpState* st, unsigned int target, long long timeout_ticks, int level, F&& last) {
-  __shared__ int timed_out;
-  unsigned int* sync = st->level_sync;
-  if (last_block(&sync[0])) {
-    last();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      sync[0] = 0U;
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-      __hip_atomic_store(&sync[1], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    return true;
-  }
-  if (threadIdx.x == 0) {
-    timed_out = 0;
-    const long long t0 = wall_clock64();
-    while (static_cast<int>(__hip_atomic_load(&sync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
-      if (wall_clock64() - t0 > timeout_ticks) {
-        timed_out = 1;
-        st->frame_failed = 1;
-        st->failed_level = level;
-        atomicAdd(&st->sync_timeouts, 1);
-        break;
-      }
-      __builtin_amdgcn_s_sleep(2);
-    }
-  }
-  __syncthreads();
-  if (timed_out != 0) return false;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  return true;
-}
-
-struct FallbackLevel {
-  Maps m;          // live maps, and the model maps the level's raycast writes
-  RayParams rp;
-  float4* pts;
-  float4* nrm;
-  uint32_t* col;
-  int tiles_x, tiles;
-  int iterations;
-  int nb;          // sweeping workgroups: the grid of the level's two-kernel trials
-};
-
-struct FallbackArgs {
-  FallbackLevel lv[kMaxLevels];
-  int levels;
-  DeviceIcpParams p;
-  IcpState* st;
-  float* partial;
-  long long timeout_ticks;
-  const uint32_t* skip_bits;
-  const uint32_t* free_bits;
-  BlockMap bm;
-};
-
-// The learned warm start's fallback pass in one launch.  A kept learned run
-// marked the frame failed, so every workgroup returns at once: the common
-// case costs one launch instead of a raycast and an ICP launch per level.
-// Otherwise, coarse to fine, each level: its preparation (one thread), the
-// raycast of every tile at the running pose (tiles strided over the grid),
-// then its trials -- the first nb workgroups sweep exactly the pixels the
-// two-kernel form's nb workgroups sweep, into the same partial rows, so the
-// reduction, the FP64 solve and the pose are bit for bit those of
-// constant-velocity tracking -- and its finish (stability check, result,
-// failure latch).  Every phase ends in a grid-wide step (above); the grid
-// (kMaxBlocks workgroups) is resident at once, checked on the host.  A
-// failed level ends the pass.
-template <int Kind, typename Store>
-__global__ void __launch_bounds__(256) fallback_pass_kernel(Sampler<Kind, Store> vol, FallbackArgs a) {
-  __shared__ float lds[4][kSums + 1];
-  __shared__ float combined[kSums + 1];
-  __shared__ float xf[24];
-  __shared__ int go;
-  IcpState* st = a.st;
-  if (st->frame_failed) return;
-  unsigned int target = __hip_atomic_load(&st->level_sync[1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  for (int l = a.levels - 1; l >= 0; --l) {
-    const FallbackLevel& L = a.lv[l];
-    if (L.iterations <= 0) continue;
-    // the preparation writes none of what the raycast reads (pose,
-    // frame_failed); the grid step below orders it before the trials
-    if (blockIdx.x == 0 && threadIdx.x == 0) icp_begin_level_state(st, l);
-    const Rigid pose = rigid_from_pose64(st->pose);
-    for (int tile = static_cast<int>(blockIdx.x); tile < L.tiles; tile += static_cast<int>(gridDim.x)) {
-      int x, y;
-      tile_pixel(tile, L.tiles_x, static_cast<int>(threadIdx.x), x, y);
-      if (x < L.rp.w && y < L.rp.h)
-        raycast_pixel(vol, L.rp, pose, x, y, L.pts, L.nrm, L.col, a.skip_bits, a.free_bits, a.bm);
-    }
-    if (!grid_step(st, ++target, a.timeout_ticks, l, [] {})) return;
-    for (int it = 0; it < L.iterations; ++it) {
-      if (threadIdx.x == 0) {
-        go = st->status == 0 ? 1 : 0;  // latched: the level is done
-        if (go != 0) {
-          for (int i = 0; i < 12; ++i) {
-            xf[i] = static_cast<float>(st->pose[i]);
-            xf[12 + i] = st->model_inv[i];
-          }
-        }
-      }
-      __syncthreads();
-      if (go == 0) break;
-      if (static_cast<int>(blockIdx.x) < L.nb) {
-        const Rigid cam = rigid_from12(xf), mod = rigid_from12(xf + 12);
-        sweep_block(L.m, cam, mod, a.p.max_dist, a.p.min_dot, a.partial, lds, static_cast<int>(blockIdx.x), L.nb);
-      }
-      const bool ok = grid_step(st, ++target, a.timeout_ticks, l, [&] {
-        combine_partials(a.partial, L.nb, combined);
-        __syncthreads();
-        if (threadIdx.x <= kSums) st->sums32[threadIdx.x] = combined[threadIdx.x];
-        if (threadIdx.x == 0) run_trial(st, a.p, combined);
-      });
-      if (!ok) return;
-    }
-    if (!grid_step(st, ++target, a.timeout_ticks, l, [&] { finish_level_block(st, a.p, l, 0.0F); })) return;
-    if (st->frame_failed) return;
-  }
-}
-
 Maps make_maps(const float4* lv_v, const float4* lv_n, int lw, int lh, const float4* md_v,
                const float4* md_n, int mw, int mh, Intr k) {
   return Maps{lv_v, lv_n, md_v, md_n, lw, lh, mw, mh, k};
@@ -1010,74 +881,6 @@ void launch_icp_reduce_once(kf_context* ctx, const float4* lv_v, const float4* l
                      rigid_from12(camera12), rigid_from12(model12), max_dist, min_dot, ctx->partials);
   hipLaunchKernelGGL(icp_combine_kernel, dim3(1), dim3(256), 0, ctx->stream, ctx->icp, ctx->partials, nb);
   KF_HIP(hipGetLastError());
-}
-
-namespace {
-
-template <int Kind, typename Store>
-bool launch_fallback_t(kf_context* ctx, const Sampler<Kind, Store>& vol, const FallbackArgs& a) {
-  static int resident = -1;  // workgroups of this instance resident at once on this device
-  if (resident < 0) {
-    int per_cu = 0, dev = 0;
-    KF_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fallback_pass_kernel<Kind, Store>, 256, 0));
-    KF_HIP(hipGetDevice(&dev));
-    hipDeviceProp_t prop{};
-    KF_HIP(hipGetDeviceProperties(&prop, dev));
-    resident = per_cu * prop.multiProcessorCount;
-  }
-  if (resident < kMaxBlocks) return false;
-  hipLaunchKernelGGL((fallback_pass_kernel<Kind, Store>), dim3(kMaxBlocks), dim3(256), 0, ctx->stream, vol, a);
-  KF_HIP(hipGetLastError());
-  return true;
-}
-
-template <int Kind>
-bool launch_fallback_kind(kf_context* ctx, const FallbackArgs& a) {
-  using Raw = typename VoxelIO<Kind>::Raw;
-  if (hashed(ctx)) {
-    const Sampler<Kind, HashedStore<Kind>> vol{
-        HashedStore<Kind>{hash_view(ctx), static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
-        1.0F / ctx->grid.voxel};
-    return launch_fallback_t(ctx, vol, a);
-  }
-  const Sampler<Kind> vol{DenseStore<Kind>{static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
-                          1.0F / ctx->grid.voxel};
-  return launch_fallback_t(ctx, vol, a);
-}
-
-}  // namespace
-
-bool launch_fallback_pass(kf_context* ctx, const DeviceIcpParams& p, const int32_t* iterations) {
-  FallbackArgs a{};
-  a.levels = ctx->cfg.levels;
-  a.p = p;
-  a.st = ctx->icp;
-  a.partial = ctx->partials;
-  a.timeout_ticks = ctx->sync_timeout_ticks;
-  a.skip_bits = ctx->skip_bits;
-  a.free_bits = ctx->free_bits;
-  a.bm = block_map(ctx->grid);
-  for (int l = 0; l < a.levels; ++l) {
-    const LevelBuffers& L = ctx->lv[l];
-    FallbackLevel& f = a.lv[l];
-    f.m = make_maps(L.vert, L.norm, L.w, L.h, L.model_pts, L.model_nrm, L.w, L.h, L.k);
-    f.rp = ray_params(ctx, l);
-    f.pts = L.model_pts;
-    f.nrm = L.model_nrm;
-    f.col = L.model_col;
-    f.tiles_x = (L.w + 15) / 16;
-    f.tiles = f.tiles_x * ((L.h + 15) / 16);
-    f.iterations = iterations[l];
-    f.nb = blocks_for(L.w * L.h);
-  }
-  switch (ctx->cfg.voxel_store) {
-    case KF_VOXEL_QUANTIZED:
-      return launch_fallback_kind<KF_VOXEL_QUANTIZED>(ctx, a);
-    case KF_VOXEL_BF16:
-      return launch_fallback_kind<KF_VOXEL_BF16>(ctx, a);
-    default:
-      return launch_fallback_kind<KF_VOXEL_FLOAT>(ctx, a);
-  }
 }
 
 static_assert(sizeof(scratch) == sizeof(float) * kMaxBlocks * kPartialStride,

@@ -542,7 +542,6 @@ void launch_icp_begin_level(kf_context* ctx, int level_index);
 // trials) as one launch that returns at once when the learned run was kept;
 // false when its grid cannot be resident at once on this device (the caller
 // then launches each level on its own).
-bool launch_fallback_pass(kf_context* ctx, const DeviceIcpParams& p, const int32_t* iterations);
 void launch_fill_voxels(kf_context* ctx);
 // skip and free bits from the observed and part bits; end_frame: the launch
 // also records the frame's pose (trajectory slot, previous poses, counter)

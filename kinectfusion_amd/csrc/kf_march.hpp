@@ -1,7 +1,6 @@
 // The raycast of one pixel (reference: SurfaceRaycast::render_pixel /
 // find_zero_crossing / write_surface_sample, include/kinectfusion/
-// raycasting.hpp:116-245), shared by the raycast kernel and the learned warm
-// start's one-launch fallback pass.  Corner blends visit the 2^3 stencil in
+// raycasting.hpp:116-245).  Corner blends visit the 2^3 stencil in
 // z -> y -> x order with the reference's weight products, so every sample is
 // bit-identical to the CPU sampler.
 #pragma once

@@ -7,7 +7,7 @@
 * RaycastOptions::seed_from_previous (raycasting.hpp:122-131): rays start
   two truncation bands in front of the previous raycast's point at the same
   pixel; maps bit-identical to the oracle given the same previous maps;
-* the learned fallback pass's bounded grid-wide wait: with its bound forced
+* the learned fallback's bounded grid-wide wait: with its bound forced
   to zero from the host, a frame that times out is reported
   (KF_ERR_DEVICE), marked untracked and not integrated, and a reset context
   afterwards reproduces a fresh context's trajectory bit for bit (the

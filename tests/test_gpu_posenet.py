@@ -233,7 +233,7 @@ def test_learned_run_still_moving_falls_back(K, PN, torch_gpu):
 
 def test_fallback_pass_equals_constant_velocity_tracking(K, PN, torch_gpu):
     """With an unreachable residual bar every learned proposal is rejected:
-    the fallback pass (one launch per level, its workgroups running the
+    the fallback (one launch per level, its workgroups running the
     trials in step) then tracks from the constant-velocity seed with the full
     schedule, and the trajectory is the constant-velocity pipeline's own (two
     launches per trial), bit for bit: the same partial rows, reduction order
