@@ -663,81 +663,143 @@ __device__ void sym_eig6(const double* eq, double* ev) {
 }
 
 // The same eigenvalues computed by one wave (the stability check at the end
-// of a level).  Each sweep is 5 rounds of 3 rotations on disjoint index
-// pairs (all 15 pairs once, round-robin order).  Lane r * 6 + c (< 36) holds
-// A[r][c]; each lane forms the rotations of its row and its column itself
-// and reads the three other entries of J^T A J it needs by shuffles, with
-// the arithmetic of B = A J, then J^T B, entry by entry; no barriers.
-// Rounding differs from the cyclic single-thread order; the eigenvalues
-// agree to ~1e-15 of the norm.
-namespace jacobi {
-constexpr int kP[5][3] = {{0, 1, 2}, {0, 3, 1}, {0, 2, 1}, {0, 1, 4}, {0, 2, 3}};
-constexpr int kQ[5][3] = {{5, 4, 3}, {4, 5, 2}, {3, 4, 5}, {2, 3, 5}, {1, 5, 4}};
-}  // namespace jacobi
+// of a level), in two steps:
+//  * lane 0 reduces the system to tridiagonal form by four Householder
+//    reflections H = I - beta v v^T, each applied as A - v w^T - w v^T with
+//    p = beta A v, K = beta (v . p) / 2, w = p - K v (every index a
+//    compile-time constant, so the matrix stays in registers);
+//  * 60 lanes multisect the six eigenvalues: the 10 lanes of eigenvalue k
+//    probe 10 interior points of its interval (initially the Gershgorin
+//    interval of the tridiagonal matrix), each counting the eigenvalues below
+//    its point by the sign changes of the Sturm sequence p_0 = 1,
+//    p_1 = d_0 - x, p_i = (d_{i-1} - x) p_{i-1} - e_{i-2}^2 p_{i-2}
+//    (division-free; magnitudes stay far inside the FP64 range for these
+//    systems), and keep the sub-interval where the count passes k.  Sixteen
+//    rounds shrink the interval by 11^16, to the rounding of the
+//    tridiagonal matrix's norm.
+// Backward stable: the eigenvalues agree with a full-precision solver to
+// ~1e-15 of the norm (a CPU emulation of these steps checks it against
+// numpy on condition numbers up to 1e8).  A non-finite system gives NaN.
+constexpr int kSectRounds = 16;
+constexpr int kSectProbes = 10;
+
+__device__ inline int sturm_count_below(const double (&d)[6], const double (&e2)[5], double x) {
+  double p_prev = 1.0, p = d[0] - x;
+  bool neg = p < 0.0 || p == 0.0;  // a zero takes the sign opposite to its predecessor (+1)
+  int changes = neg ? 1 : 0;
+#pragma unroll
+  for (int i = 1; i < 6; ++i) {
+    const double p_next = (d[i] - x) * p - e2[i - 1] * p_prev;
+    const bool n_neg = p_next == 0.0 ? !neg : p_next < 0.0;
+    changes += n_neg != neg ? 1 : 0;
+    neg = n_neg;
+    p_prev = p;
+    p = p_next;
+  }
+  return changes;
+}
 
 __device__ void sym_eig6_wave(const double* eq, double* ev) {
+  __shared__ double tri[11];  // d[6], e[5]
   const int lane = threadIdx.x & 63;
-  const bool on = lane < 36;
-  const int r = on ? lane / 6 : 0, c = on ? lane % 6 : 0;
-  const int lo = min(r, c), hi = max(r, c);
-  double a = on ? eq[lo * 6 - (lo * (lo - 1)) / 2 + (hi - lo)] : 0.0;  // packed upper triangle, row by row
-  for (int sweep = 0; sweep < 32; ++sweep) {
-    double off = on && r < c ? a * a : 0.0, diag = on && r == c ? a * a : 0.0;
-    for (int o = 32; o > 0; o >>= 1) {
-      off += __shfl_xor(off, o);
-      diag += __shfl_xor(diag, o);
-    }
-    if (off <= 1.0e-32 * diag) break;
-    for (int round = 0; round < 5; ++round) {
-      // the rotation holding index j: its partner, whether j is its first index
-      auto find = [&](int j, int& other, bool& first) {
-        other = 0;
-        first = true;
-        for (int i = 0; i < 3; ++i) {
-          if (jacobi::kP[round][i] == j) { other = jacobi::kQ[round][i]; first = true; }
-          if (jacobi::kQ[round][i] == j) { other = jacobi::kP[round][i]; first = false; }
-        }
-      };
-      int orow, ocol;
-      bool frow, fcol;
-      find(r, orow, frow);
-      find(c, ocol, fcol);
-      const int pr = frow ? r : orow, qr = frow ? orow : r, pc = fcol ? c : ocol, qc = fcol ? ocol : c;
-      const double app_r = __shfl(a, pr * 7), aqq_r = __shfl(a, qr * 7), apq_r = __shfl(a, pr * 6 + qr);
-      const double app_c = __shfl(a, pc * 7), aqq_c = __shfl(a, qc * 7), apq_c = __shfl(a, pc * 6 + qc);
-      const double a_r_oc = __shfl(a, r * 6 + ocol), a_or_c = __shfl(a, orow * 6 + c),
-                   a_or_oc = __shfl(a, orow * 6 + ocol);
-      // The angle (t = tan) takes the hardware reciprocal and square root:
-      // any t gives an orthogonal rotation through c = 1 / sqrt(1 + t^2),
-      // s = t c, and an angle off by a few ulp leaves a remainder of that
-      // order in A[p][q] for the next sweep.  c itself is full precision
-      // (reciprocal square root refined by three Newton steps).
-      auto rotation = [](double app, double aqq, double apq, double& cc, double& ss) {
-        cc = 1.0;
-        ss = 0.0;
-        if (apq != 0.0) {
-          const double theta = (aqq - app) * __builtin_amdgcn_rcp(2.0 * apq);
-          const double t = (theta >= 0.0 ? 1.0 : -1.0) *
-                           __builtin_amdgcn_rcp(fabs(theta) + __builtin_amdgcn_sqrt(theta * theta + 1.0));
-          const double u = t * t + 1.0;
-          double y = __builtin_amdgcn_rsq(u);
+  if (lane == 0) {
+    double a[6][6];
+    {
+      int idx = 0;
 #pragma unroll
-          for (int k = 0; k < 3; ++k) y = y * (1.5 - (0.5 * u) * (y * y));
-          cc = y;
-          ss = t * y;
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int c = r; c < 6; ++c, ++idx) {
+          a[r][c] = eq[idx];
+          a[c][r] = eq[idx];
         }
-      };
-      double cr, sr, cc, sc;
-      rotation(app_r, aqq_r, apq_r, cr, sr);
-      rotation(app_c, aqq_c, apq_c, cc, sc);
-      // B = A J at (r, c) and (orow, c), then A = J^T B at (r, c)
-      const double b_r = fcol ? cc * a - sc * a_r_oc : sc * a_r_oc + cc * a;
-      const double b_o = fcol ? cc * a_or_c - sc * a_or_oc : sc * a_or_oc + cc * a_or_c;
-      a = frow ? cr * b_r - sr * b_o : sr * b_o + cr * b_r;
     }
+    double d[6], e[5];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      double n2 = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < 6; ++i) n2 += a[i][k] * a[i][k];
+      double alpha = sqrt(n2);
+      if (a[k + 1][k] > 0.0) alpha = -alpha;
+      double v[6];
+#pragma unroll
+      for (int i = k + 1; i < 6; ++i) v[i] = a[i][k];
+      v[k + 1] -= alpha;
+      double vn = 0.0;
+#pragma unroll
+      for (int i = k + 1; i < 6; ++i) vn += v[i] * v[i];
+      e[k] = alpha;
+      d[k] = a[k][k];
+      if (vn > 0.0) {
+        const double beta = 2.0 / vn;
+        double pv[6], w[6];
+        double kk = 0.0;
+#pragma unroll
+        for (int i = k + 1; i < 6; ++i) {
+          double acc = 0.0;
+#pragma unroll
+          for (int j = k + 1; j < 6; ++j) acc += a[i][j] * v[j];
+          pv[i] = beta * acc;
+          kk += v[i] * pv[i];
+        }
+        kk *= 0.5 * beta;
+#pragma unroll
+        for (int i = k + 1; i < 6; ++i) w[i] = pv[i] - kk * v[i];
+#pragma unroll
+        for (int i = k + 1; i < 6; ++i)
+#pragma unroll
+          for (int j = k + 1; j < 6; ++j) a[i][j] -= v[i] * w[j] + w[i] * v[j];
+      }
+    }
+    d[4] = a[4][4];
+    d[5] = a[5][5];
+    e[4] = a[5][4];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) tri[i] = d[i];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) tri[6 + i] = e[i];
   }
-  const double d = __shfl(a, (lane < 6 ? lane : 0) * 7);
-  if (lane < 6) ev[lane] = d;
+  // lane 0's LDS writes precede every lane's reads in the wave's program order
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  double d[6], e2[5];
+  bool finite = true;
+  double lo = 1.0e308, hi = -1.0e308;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    d[i] = tri[i];
+    finite = finite && isfinite(d[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 5; ++i) {
+    const double ei = tri[6 + i];
+    finite = finite && isfinite(ei);
+    e2[i] = ei * ei;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {  // Gershgorin discs of the tridiagonal matrix
+    const double r = (i > 0 ? fabs(tri[5 + i]) : 0.0) + (i < 5 ? fabs(tri[6 + i]) : 0.0);
+    lo = fmin(lo, d[i] - r);
+    hi = fmax(hi, d[i] + r);
+  }
+  const double pad = (hi - lo) * 1.0e-12 + 1.0e-300;
+  lo -= pad;
+  hi += pad;
+  const int k = lane / kSectProbes, j = lane % kSectProbes + 1;  // eigenvalue, probe (lanes < 60)
+  for (int round = 0; round < kSectRounds; ++round) {
+    const double step = (hi - lo) * (1.0 / (kSectProbes + 1));
+    const bool above = k < 6 && sturm_count_below(d, e2, lo + step * j) > k;
+    const unsigned long long votes = __ballot(above);
+    const unsigned group = k < 6 ? static_cast<unsigned>((votes >> (kSectProbes * k)) & 0x3FFULL) : 0U;
+    const int first = group != 0U ? __builtin_ctz(group) + 1 : kSectProbes + 1;  // first probe above
+    const double nlo = first > 1 ? lo + step * (first - 1) : lo;
+    const double nhi = first <= kSectProbes ? lo + step * first : hi;
+    lo = nlo;
+    hi = nhi;
+  }
+  if (k < 6 && j == 1) ev[k] = finite ? 0.5 * (lo + hi) : __builtin_nan("");
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
 

@@ -297,8 +297,9 @@ def test_icp_track_level_against_oracle(K, ko, scene, torch_gpu):
     assert out.ok and r.status in (0, 1)
     assert out.diagnostics.correspondences == r.correspondences
     assert np.abs(out.pose - ref_pose).max() < 1e-6
-    # the stability spectrum: one-wave round-robin Jacobi on the device, cyclic
-    # Jacobi in the oracle (both FP64; the systems agree to float rounding)
+    # the stability spectrum: Householder tridiagonalisation and Sturm
+    # multisection in one wave on the device, cyclic Jacobi in the oracle
+    # (both FP64; the systems agree to float rounding)
     d = out.diagnostics
     assert r.min_eigenvalue > 0.0 and r.condition_number > 1.0
     assert abs(d.min_eigenvalue - r.min_eigenvalue) <= 1e-4 * r.min_eigenvalue
