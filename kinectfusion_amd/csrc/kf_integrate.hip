@@ -340,7 +340,10 @@ __device__ inline bool same_raw(Raw a, Raw b) {
 // from the boundaries the approximate value decides the same way as the exact
 // one, so the outcome is bit-identical:
 //  * the pixel: lround(u) = floor(u' + 0.5) when u' is farther than
-//    1e-3 + 2e-6 |q'| from a half-integer (|u'| < 4096, q' = fx x / z);
+//    3e-4 + 2e-6 |q'| from a half-integer (q' = fx x / z): u' differs from
+//    the exact float u by less than 2e-6 |q'| (the reciprocal and two
+//    roundings) plus the float rounding of both (below 1.3e-4 wherever the
+//    pixel can be in the image, |u| < 1024); farther out both land outside;
 //  * sd >= truncation (free space): tsdf = clamp(sd / trunc) = 1 exactly and
 //    no colour; the update ((t0 w0) + (1 w)) / (w0 + w) is 1 exactly when
 //    t0 = 1 (numerator and denominator are the same rounded sum), so the
@@ -385,19 +388,19 @@ __device__ inline void fuse_batch(const Grid& g, const FrameParams& f, const Rig
     const float rz = __builtin_amdgcn_rcpf(cp[k].z);
     const float qx = (f.k.fx * cp[k].x) * rz, qy = (f.k.fy * cp[k].y) * rz;
     const float ua = qx + f.k.cx, va = qy + f.k.cy;
-    bool exact = !(fabsf(ua) < 4096.0F && fabsf(va) < 4096.0F);
-    if (!exact) {
-      const float eu = ua - floorf(ua), ev = va - floorf(va);
-      exact = fabsf(eu - 0.5F) <= 1.0e-3F + 2.0e-6F * fabsf(qx) || fabsf(ev - 0.5F) <= 1.0e-3F + 2.0e-6F * fabsf(qy);
-    }
+    // the nearest pixel; u' lies within the margin of a half-integer when
+    // |u' - n| >= 0.5 - margin (a NaN projection fails the test too)
+    const float nu = floorf(ua + 0.5F), nv = floorf(va + 0.5F);
+    const bool fast = fabsf(ua - nu) < (0.5F - 3.0e-4F) - 2.0e-6F * fabsf(qx) &&
+                      fabsf(va - nv) < (0.5F - 3.0e-4F) - 2.0e-6F * fabsf(qy);
     int col, row;
-    if (exact) {
+    if (!fast) {
       float u, v;
       project(f.k, cp[k], u, v);
       if (!round_index(u, f.w, col) || !round_index(v, f.h, row)) continue;
     } else {
-      col = static_cast<int>(floorf(ua + 0.5F));
-      row = static_cast<int>(floorf(va + 0.5F));
+      col = static_cast<int>(nu);
+      row = static_cast<int>(nv);
       if (col < 0 || col >= f.w || row < 0 || row >= f.h) continue;
     }
     pi[k] = row * f.w + col;

@@ -7,7 +7,8 @@ s' = sqrt(x.x) * rcp(lambda) with the hardware reciprocal and square root
 quotients), and uses them only when they are farther than a margin from a
 decision boundary.  Here the approximations are pushed to the worst case
 (every relative error in [-2^-21, 2^-21], and 4x that), and every decision
-the margins accept must equal the exact one: the rounded pixel index, and
+the margins accept must equal the exact one: the rounded pixel index where it
+lies in the image (outside it, both paths skip the voxel), and
 sd >= truncation / sd < -truncation."""
 import numpy as np
 
@@ -29,18 +30,48 @@ def test_pixel_rounding_margin():
     q = (a / z).astype(F)
     u = (q + cx).astype(F)
     exact = lround_half_away(u.astype(np.float64))
+    width = 640
     for scale in (1.0, 4.0):
         for e in (-REL * scale, REL * scale, 0.0):
             qa = (q.astype(np.float64) * (1.0 + e)).astype(F)
             ua = (qa + cx).astype(F)
-            ok = np.abs(ua) < 4096
-            frac = ua - np.floor(ua)
-            safe = ok & (np.abs(frac - F(0.5)) > F(1e-3) + F(2e-6) * np.abs(qa))
-            fast = np.floor(ua + F(0.5))
-            assert np.array_equal(fast[safe], exact[safe])
+            nu = np.floor((ua + F(0.5)).astype(F))
+            with np.errstate(invalid="ignore"):
+                safe = np.abs((ua - nu).astype(F)) < (F(0.5) - F(3e-4)) - F(2e-6) * np.abs(qa)
+            # the decision that matters: the pixel index when it lies in the
+            # image, or "outside" (both paths then skip the voxel)
+            fast_pix = np.where((nu >= 0) & (nu < width), nu, -1)
+            exact_pix = np.where((exact >= 0) & (exact < width), exact, -1)
+            assert np.array_equal(fast_pix[safe], exact_pix[safe])
             # almost every voxel that lands near the image takes the fast path
             near = np.abs(ua) < 1000
-            assert safe[near].mean() > 0.99
+            assert safe[near].mean() > 0.997
+
+
+def test_pixel_rounding_margin_near_half_integers():
+    """Projections aimed within 1e-3 of a half-integer, across the image and
+    its borders (u near -0.5 and 639.5), with the worst-case approximation
+    errors."""
+    rng = np.random.default_rng(5)
+    n = 400_000
+    fx, cx, width = F(525.0), F(319.5), 640
+    z = rng.uniform(0.05, 5, n).astype(F)
+    target = rng.integers(-3, width + 3, n) + 0.5 + rng.uniform(-1e-3, 1e-3, n)
+    x = ((target - float(cx)) * z / float(fx)).astype(F)
+    q = ((fx * x).astype(F) / z).astype(F)
+    u = (q + cx).astype(F)
+    exact = lround_half_away(u.astype(np.float64))
+    exact_pix = np.where((exact >= 0) & (exact < width), exact, -1)
+    taken = 0
+    for e in (-4 * REL, -REL, 0.0, REL, 4 * REL):
+        qa = (q.astype(np.float64) * (1.0 + e)).astype(F)
+        ua = (qa + cx).astype(F)
+        nu = np.floor((ua + F(0.5)).astype(F))
+        safe = np.abs((ua - nu).astype(F)) < (F(0.5) - F(3e-4)) - F(2e-6) * np.abs(qa)
+        fast_pix = np.where((nu >= 0) & (nu < width), nu, -1)
+        assert np.array_equal(fast_pix[safe], exact_pix[safe])
+        taken += int(safe.sum())
+    assert taken > 0
 
 
 def test_signed_distance_margins():
