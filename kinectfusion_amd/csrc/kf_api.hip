@@ -607,6 +607,13 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
       ctx->part_bits = dalloc<uint32_t>(ctx->part_words);
       ctx->free_bits = dalloc<uint32_t>(ctx->block_words);
       KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
+      if (c.storage == KF_STORAGE_HASHED) {
+        const BlockMap cm = coarse_map(ctx->grid);
+        ctx->coarse_bits = dalloc<uint32_t>((static_cast<size_t>(cm.bx) * cm.by * cm.bz + 31) / 32);
+        const size_t nb = static_cast<size_t>(bm.bx) * bm.by * bm.bz;
+        ctx->nbr_obs = dalloc<uint32_t>(nb);
+        KF_HIP(hipMemsetAsync(ctx->nbr_obs, 0, nb * 4, ctx->stream));
+      }
       if (c.storage == KF_STORAGE_DENSE && c.integration_mode == KF_INTEGRATION_BAND) {
         ctx->band_bits = dalloc<uint32_t>(ctx->block_words);
         KF_HIP(hipMemsetAsync(ctx->band_bits, 0, ctx->block_words * 4, ctx->stream));
@@ -673,6 +680,8 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->skip_bits);
   dfree(ctx->part_bits);
   dfree(ctx->free_bits);
+  dfree(ctx->coarse_bits);
+  dfree(ctx->nbr_obs);
   dfree(ctx->counters);
   dfree(ctx->fused);
   dfree(ctx->partials);

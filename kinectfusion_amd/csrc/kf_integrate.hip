@@ -752,6 +752,54 @@ __global__ void __launch_bounds__(64) skip_map_rows_kernel(const uint32_t* __res
   free_bits[t] = interior ? full & ((full << 1) | full_lo) & ((full >> 1) | (full_hi << 31)) : 0U;
 }
 
+// coarse bit of a cell of 4^3 blocks = every block of it (inside the grid)
+// is a skip block.  One thread per cell; a wave's 64 consecutive cells form
+// two words.
+__global__ void __launch_bounds__(256) coarse_skip_kernel(const uint32_t* __restrict__ skip,
+                                                           uint32_t* __restrict__ coarse, BlockMap bm, BlockMap cm,
+                                                           int ncells) {
+  const int c = static_cast<int>(blockIdx.x * 256 + threadIdx.x);
+  bool all = false;
+  if (c < ncells) {
+    const int cx = c % cm.bx, cy = (c / cm.bx) % cm.by, cz = c / (cm.bx * cm.by);
+    all = true;
+    for (int z = cz * 4; z < min(cz * 4 + 4, bm.bz) && all; ++z)
+      for (int y = cy * 4; y < min(cy * 4 + 4, bm.by) && all; ++y)
+        for (int x = cx * 4; x < min(cx * 4 + 4, bm.bx); ++x)
+          if (!bm.bit(skip, bm.flat(x, y, z))) {
+            all = false;
+            break;
+          }
+  }
+  const unsigned long long m = __ballot(all);
+  const int lane = threadIdx.x & 63;
+  const int word0 = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 5;
+  const int nwords = (ncells + 31) >> 5;
+  if (lane == 0 && word0 < nwords) coarse[word0] = static_cast<uint32_t>(m);
+  if (lane == 32 && word0 + 1 < nwords) coarse[word0 + 1] = static_cast<uint32_t>(m >> 32);
+}
+
+// Neighbour masks of the non-skip blocks: bit (dz + 1) * 9 + (dy + 1) * 3 +
+// (dx + 1) = the block at that offset lies in the grid and holds an observed
+// voxel.  A skip block keeps its old mask: the raycaster uses a mask only for
+// a non-skip block.  One thread per block.
+__global__ void __launch_bounds__(256) neighbour_mask_kernel(const uint32_t* __restrict__ obs,
+                                                              const uint32_t* __restrict__ skip,
+                                                              uint32_t* __restrict__ nbr, BlockMap bm, int nblocks) {
+  const int b = static_cast<int>(blockIdx.x * 256 + threadIdx.x);
+  if (b >= nblocks || bm.bit(skip, b)) return;
+  const int x = b % bm.bx, y = (b / bm.bx) % bm.by, z = b / (bm.bx * bm.by);
+  uint32_t m = 0U;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = x + dx, yy = y + dy, zz = z + dz;
+        if (xx < 0 || yy < 0 || zz < 0 || xx >= bm.bx || yy >= bm.by || zz >= bm.bz) continue;
+        if (bm.bit(obs, bm.flat(xx, yy, zz))) m |= 1U << ((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+      }
+  nbr[b] = m;
+}
+
 // observed bits from the voxel array (after an upload): one thread per block.
 template <int Kind>
 __global__ void __launch_bounds__(256) rebuild_observed_kernel(const typename VoxelIO<Kind>::Raw* __restrict__ v,
@@ -1207,6 +1255,15 @@ void launch_skip_map(kf_context* ctx, bool end_frame) {
                        ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks, end_frame ? ctx->icp : nullptr,
                        ctx->trajectory, keep);
   }
+  if (ctx->coarse_bits != nullptr) {
+    const BlockMap cm = coarse_map(ctx->grid);
+    const int ncells = cm.bx * cm.by * cm.bz;
+    hipLaunchKernelGGL(coarse_skip_kernel, dim3((ncells + 255) / 256), dim3(256), 0, ctx->stream, ctx->skip_bits,
+                       ctx->coarse_bits, bm, cm, ncells);
+  }
+  if (ctx->nbr_obs != nullptr)
+    hipLaunchKernelGGL(neighbour_mask_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
+                       ctx->skip_bits, ctx->nbr_obs, bm, nblocks);
   KF_HIP(hipGetLastError());
 }
 

@@ -453,6 +453,12 @@ struct kf_context {
   // blocks without loads)
   uint32_t* part_bits = nullptr;
   uint32_t* free_bits = nullptr;
+  // hashed storage: one bit per cell of 4^3 blocks, set when every block of
+  // the cell is a skip block (the raycaster steps over such a cell at once)
+  uint32_t* coarse_bits = nullptr;
+  // hashed storage: per non-skip block, bit (dz + 1) * 9 + (dy + 1) * 3 +
+  // (dx + 1) set when the block at that offset holds an observed voxel
+  uint32_t* nbr_obs = nullptr;
   size_t part_words = 0;
   size_t block_words = 0;
   uint32_t* counters = nullptr;   // [0] brick count, [1..2] fused counts (u64 pair as 2x u32), spare
@@ -562,6 +568,11 @@ struct BlockMap {
 };
 inline BlockMap block_map(const Grid& g) {
   return BlockMap{(g.nx + 7) / 8, (g.ny + 7) / 8, (g.nz + 7) / 8};
+}
+// cells of 4^3 blocks (32^3 voxels)
+inline BlockMap coarse_map(const Grid& g) {
+  const BlockMap b = block_map(g);
+  return BlockMap{(b.bx + 3) / 4, (b.by + 3) / 4, (b.bz + 3) / 4};
 }
 DeviceIcpParams to_device_params(const kf_icp_options& o);
 

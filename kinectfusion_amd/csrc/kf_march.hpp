@@ -23,12 +23,13 @@ __device__ inline uint32_t pixel_from_color(V3 c) {
   return to_byte(c.x) | (to_byte(c.y) << 8U) | (to_byte(c.z) << 16U) | (0xFFU << 24U);
 }
 
-// One axis of a block's safe interior: block b covers grid coordinates
-// [8b, min(8b + 8, n)); the ray's coordinate o + d * len lies at least
-// `safe` inside it for len in [enter, exit), intersected into the running
-// interval.  A ray (almost) parallel to the axis keeps its coordinate.
-__device__ inline void block_span(int b, float o, float d, int n, float safe, float& enter, float& exit) {
-  const float lo = static_cast<float>(b * 8) + safe, hi = static_cast<float>(min(b * 8 + 8, n)) - safe;
+// One axis of a box's safe interior: the box covers grid coordinates
+// [first, min(first + size, n)); the ray's coordinate o + d * len lies at
+// least `safe` inside it for len in [enter, exit), intersected into the
+// running interval.  A ray (almost) parallel to the axis keeps its coordinate.
+__device__ inline void box_span(int first, int size, float o, float d, int n, float safe, float& enter,
+                                float& exit) {
+  const float lo = static_cast<float>(first) + safe, hi = static_cast<float>(min(first + size, n)) - safe;
   if (fabsf(d) <= 1.0e-12F) {
     if (o < lo || o > hi) exit = -3.0e38F;
     return;
@@ -44,7 +45,9 @@ template <int Kind, typename Store>
 __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayParams& rp, const Rigid& pose, int x,
                                      int y, float4* __restrict__ pts, float4* __restrict__ nrm,
                                      uint32_t* __restrict__ col, const uint32_t* __restrict__ skip_bits,
-                                     const uint32_t* __restrict__ free_bits, const BlockMap& bm) {
+                                     const uint32_t* __restrict__ free_bits,
+                                     const uint32_t* __restrict__ nbr_obs, const BlockMap& bm,
+                                     const uint32_t* __restrict__ coarse_bits, const BlockMap& cm) {
   const size_t i = static_cast<size_t>(y) * rp.w + x;
   // RaycastOptions::seed_from_previous (raycasting.hpp:122-131): start just
   // in front of the surface this pixel found in the previous raycast into
@@ -114,7 +117,9 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
     while (static_cast<double>(len) < len_enter && len <= rp.max_depth) len += base_step;
     const float free_step = fmax_ref(base_step, (1.0F * vol.g.trunc) * rp.step_scale);
     int cur_blk = -1;
-    bool cur_skip = false, cur_free = false;
+    bool cur_skip = false, cur_free = false, cur_empty = false;
+    uint32_t cur_nbr = 0U;  // observed blocks among the current block's 3^3 (hashed store)
+    SlotCache slots;  // the last block a sample probed (hashed store)
     // The samples of a skip or free block that lie deeper than kSafe voxels
     // inside it on every axis are stepped in a tight loop (a float add and a
     // compare each): [enter, exit) is that stretch of ray lengths, from the
@@ -124,7 +129,26 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
     // voxels of the grid origin), and the interval's own rounding adds as
     // little, so every sample in it falls in the block as the per-sample
     // path would classify it.
+    //
+    // A block that holds no observed voxel but has an observed neighbour
+    // ("empty") is no skip block, yet a sample whose 2^3 stencil lies inside
+    // it reads no observed corner either: the samples deeper than 0.5 + kSafe
+    // voxels inside it (the stencil base floor(g - 0.5) and its +1 corner
+    // then stay in the block) are stepped the same way, each a sample
+    // without TSDF.  nbr_obs holds, per block, which of its 3^3 blocks hold
+    // an observed voxel; a sample of an empty block whose stencil reaches
+    // only unobserved blocks is a sample without TSDF as well, decided
+    // without a read.
+    //
+    // coarse_bits (optional) marks cells of 4^3 blocks that are all skip
+    // blocks: a skip block in such a cell is stepped over the whole cell's
+    // safe interior at once, so a ray crossing unobserved space reads one
+    // class per 32^3-voxel cell instead of one per block.
     constexpr float kSafe = 0.05F;
+    // Empty blocks pay where a sample costs a hash probe before its corner
+    // reads; a direct store reads the corners at once, and the extra class
+    // word per block was slower there (-1.4 % headline frame rate).
+    constexpr bool kEmptyClass = !Store::kDirect;
     float enter = 0.0F, exit = -1.0F;
     while (len <= rp.max_depth) {
       if (len > len_stop) break;
@@ -135,22 +159,33 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
         const int gx = static_cast<int>(gc.x) >> 3, gy = static_cast<int>(gc.y) >> 3, gz = static_cast<int>(gc.z) >> 3;
         const int fb = bm.flat(gx, gy, gz);
         if (fb != cur_blk) {
-          // both words in one round trip: the class reads both unconditionally
-          // (a block is never both skip and free); with the free word read
-          // only for a non-skip block, each new block cost two dependent
-          // reads (1.5 % of the frame rate)
+          // the class words in one round trip, read unconditionally; with
+          // the free word read only for a non-skip block, each new block
+          // cost two dependent reads (1.5 % of the frame rate)
           cur_blk = fb;
           const uint32_t sh = static_cast<uint32_t>(fb & 31);
-          const uint32_t cls = ((skip_bits[fb >> 5] >> sh) & 1U) | (((free_bits[fb >> 5] >> sh) & 1U) << 1);
-          cur_skip = cls == 1U;
-          cur_free = cls == 2U;
-          if (cls != 0U) {
+          // a skip block holds no observed voxel, a free block only
+          // observed ones: the three bits give four exclusive classes
+          const uint32_t skip = (skip_bits[fb >> 5] >> sh) & 1U, fre = (free_bits[fb >> 5] >> sh) & 1U;
+          if constexpr (kEmptyClass) cur_nbr = nbr_obs[fb];
+          const uint32_t seen = kEmptyClass ? (cur_nbr >> 13) & 1U : 1U;
+          bool coarse = false;
+          if (coarse_bits != nullptr) {
+            const int cf = cm.flat(gx >> 2, gy >> 2, gz >> 2);
+            coarse = ((coarse_bits[cf >> 5] >> (cf & 31)) & 1U) != 0U;
+          }
+          cur_skip = skip != 0U;
+          cur_free = fre != 0U;
+          cur_empty = (skip | seen) == 0U;
+          if (cur_skip || cur_free || cur_empty) {
             const V3 go = vol.grid_coords(pose.t), gd = scale3(dir, vol.inv_voxel);
+            const float margin = cur_empty ? 0.5F + kSafe : kSafe;
+            const int size = coarse ? 32 : 8, sh3 = coarse ? 5 : 3;
             enter = -3.0e38F;
             exit = 3.0e38F;
-            block_span(gx, go.x, gd.x, vol.g.nx, kSafe, enter, exit);
-            block_span(gy, go.y, gd.y, vol.g.ny, kSafe, enter, exit);
-            block_span(gz, go.z, gd.z, vol.g.nz, kSafe, enter, exit);
+            box_span((gx << 3) >> sh3 << sh3, size, go.x, gd.x, vol.g.nx, margin, enter, exit);
+            box_span((gy << 3) >> sh3 << sh3, size, go.y, gd.y, vol.g.ny, margin, enter, exit);
+            box_span((gz << 3) >> sh3 << sh3, size, go.z, gd.z, vol.g.nz, margin, enter, exit);
           }
         }
         if (cur_skip) {  // no corner is observed: no TSDF
@@ -174,9 +209,35 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
           prev_p = add3(pose.t, scale3(dir, last));  // the last free sample's point
           continue;
         }
+        if (cur_empty && len >= enter && len < exit) {  // a stencil inside an empty block: no TSDF
+          have_prev = false;
+          len += base_step;
+          while (len < exit && len <= rp.max_depth) len += base_step;
+          continue;
+        }
+        if (cur_empty) {
+          // the stencil's corners c0 and c0 + 1 per axis (c0 = floor(g - 0.5),
+          // as the sampler computes it) lie in the blocks -1, 0 or +1 from
+          // this one; bit (dz + 1) * 9 + (dy + 1) * 3 + (dx + 1) of `reach`
+          // marks each block the stencil reaches
+          auto axis = [](float g, int b) {
+            const int c0 = static_cast<int>(floorf(g - 0.5F));
+            return (1U << ((c0 >> 3) - b + 1)) | (1U << (((c0 + 1) >> 3) - b + 1));
+          };
+          const uint32_t xs = axis(gc.x, gx), ys = axis(gc.y, gy), zs = axis(gc.z, gz);
+          const uint32_t yx = ((ys & 1U) != 0U ? xs : 0U) | ((ys & 2U) != 0U ? xs << 3 : 0U) |
+                              ((ys & 4U) != 0U ? xs << 6 : 0U);
+          const uint32_t reach = ((zs & 1U) != 0U ? yx : 0U) | ((zs & 2U) != 0U ? yx << 9 : 0U) |
+                                 ((zs & 4U) != 0U ? yx << 18 : 0U);
+          if ((reach & cur_nbr) == 0U) {  // no observed corner: no TSDF
+            have_prev = false;
+            len += base_step;
+            continue;
+          }
+        }
       }
       float t;
-      if (!vol.tsdf(point, rp.require_all != 0, t)) {
+      if (!vol.tsdf(point, rp.require_all != 0, t, slots)) {
         have_prev = false;
         len += base_step;
         continue;

@@ -106,6 +106,55 @@ struct HashedStore {
     return color_cached(g, x, y, z, cache);
   }
   __device__ inline bool has_colors() const { return cpool != nullptr; }
+  // The pool slots of the eight corners (x fastest) of the 2^3 stencil with
+  // base corner (x, y, z); bit c of `in` marks a corner inside the grid.
+  // The first probe of every corner outside the cached block is issued
+  // before any is examined, so a stencil across block faces costs one round
+  // of table reads, not one per block; a probe that meets another key walks
+  // the chain.  The cache then holds the block of corner `own`.
+  __device__ inline void slots8(int x, int y, int z, unsigned in, int own, uint32_t (&slot)[8],
+                                SlotCache& cache) const {
+    uint32_t key[8];
+    uint2 e[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      key[c] = hv.key_of(x + (c & 1), y + ((c >> 1) & 1), z + (c >> 2));
+      e[c] = make_uint2(0xFFFFFFFFU, 0xFFFFFFFFU);  // kHashEmpty
+      if (((in >> c) & 1U) != 0U && key[c] != cache.key)
+        e[c] = reinterpret_cast<const uint2*>(hv.table)[hv.bucket(key[c])];
+    }
+    unsigned walk = 0U;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      uint32_t sl = kNoSlot;
+      if (((in >> c) & 1U) != 0U) {
+        if (key[c] == cache.key) {
+          sl = cache.slot;
+        } else if (e[c].y == key[c]) {
+          sl = e[c].x == kPendingSlot ? kNoSlot : e[c].x;
+        } else if (e[c].x != 0xFFFFFFFFU || e[c].y != 0xFFFFFFFFU) {
+          walk |= 1U << c;  // the home bucket holds another key
+        }
+      }
+      slot[c] = sl;
+    }
+    if (walk != 0U) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (((walk >> c) & 1U) != 0U) slot[c] = hv.find(key[c]);
+    }
+    if (((in >> own) & 1U) != 0U) {
+      uint32_t k = 0U, sl = 0U;
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+        if (c == own) {
+          k = key[c];
+          sl = slot[c];
+        }
+      cache.key = k;
+      cache.slot = sl;
+    }
+  }
 };
 
 template <int Kind, typename Store = DenseStore<Kind>>
@@ -155,6 +204,27 @@ struct Sampler {
         inb = 0xFFU;
         return;
       }
+    } else {
+      // the eight slots first (their probes together), then the voxels
+      inb = 0U;
+      if (st.ok) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int x = st.bx + (c & 1), y = st.by + ((c >> 1) & 1), z = st.bz + (c >> 2);
+          if (st.interior || (x >= 0 && y >= 0 && z >= 0 && x < g.nx && y < g.ny && z < g.nz)) inb |= 1U << c;
+        }
+      }
+      // the corner holding the stencil's point: base + 1 where the fraction is >= 0.5
+      const int own = (st.fr.x >= 0.5F ? 1 : 0) | (st.fr.y >= 0.5F ? 2 : 0) | (st.fr.z >= 0.5F ? 4 : 0);
+      uint32_t slot[8];
+      s.slots8(st.bx, st.by, st.bz, inb, own, slot, cache);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int x = st.bx + (c & 1), y = st.by + ((c >> 1) & 1), z = st.bz + (c >> 2);
+        raw[c] = slot[c] == kNoSlot ? IO::fresh()
+                                    : s.pool[static_cast<size_t>(slot[c]) * kBlockVoxels + intra_of(x, y, z)];
+      }
+      return;
     }
     inb = 0U;
 #pragma unroll
@@ -222,55 +292,27 @@ struct Sampler {
 
   // TrilinearSampler::tsdf + blend_tsdf.
   __device__ bool tsdf(V3 p, bool require_all, float& out) const {
+    SlotCache cache;
+    return tsdf(p, require_all, out, cache);
+  }
+  // The same with a block-slot cache that the caller keeps across samples: a
+  // hashed store probes a block once while consecutive samples stay in it
+  // (the table does not change during a launch).
+  __device__ bool tsdf(V3 p, bool require_all, float& out, SlotCache& cache) const {
+    const Stencil st = stencil(p);
+    if (!st.ok) return false;
+    typename IO::Raw raw[8];
     if constexpr (Store::kDirect) {
       // the eight corners as four x-pair loads; blend_all() and blend() are
-      // the same float sequence as the loop below
-      const Stencil st = stencil(p);
-      if (!st.ok) return false;
-      typename IO::Raw raw[8];
+      // the same float sequence
       if (st.interior) {
         s.cell(g, st.bx, st.by, st.bz, raw);
         return blend_all(st, raw, require_all, out);
       }
-      unsigned inb;
-      SlotCache cache;
-      load8(st, raw, inb, cache);
-      return blend(st, raw, inb, require_all, out);
     }
-    const V3 gc = grid_coords(p);
-    if (!contains(gc)) return false;
-    const V3 gg = sub3(gc, mk3(0.5F, 0.5F, 0.5F));
-    const V3 fl = mk3(floorf(gg.x), floorf(gg.y), floorf(gg.z));
-    const int bx = static_cast<int>(fl.x), by = static_cast<int>(fl.y), bz = static_cast<int>(fl.z);
-    const V3 fr = sub3(gg, fl);
-    const bool interior = bx >= 0 && by >= 0 && bz >= 0 && bx + 1 < g.nx && by + 1 < g.ny &&
-                          bz + 1 < g.nz;
-    float acc = 0.0F, wsum = 0.0F;
-    int valid = 0;
-    SlotCache cache;
-#pragma unroll
-    for (int oz = 0; oz <= 1; ++oz)
-#pragma unroll
-      for (int oy = 0; oy <= 1; ++oy)
-#pragma unroll
-        for (int ox = 0; ox <= 1; ++ox) {
-          const int x = bx + ox, y = by + oy, z = bz + oz;
-          if (!interior && (x < 0 || y < 0 || z < 0 || x >= g.nx || y >= g.ny || z >= g.nz)) continue;
-          float t, w;
-          bool obs, fin;
-          IO::unpack(s.voxel_cached(g, x, y, z, cache), t, w, obs, fin);
-          if (!obs || !fin) continue;
-          const float cw = axis_w(fr.x, ox) * axis_w(fr.y, oy) * axis_w(fr.z, oz);
-          acc += cw * t;
-          wsum += cw;
-          ++valid;
-        }
-    if (require_all && valid < 8) return false;
-    if (wsum <= 1.0e-6F) return false;
-    const float d = acc / wsum;
-    if (!isfinite(d)) return false;
-    out = d;
-    return true;
+    unsigned inb;
+    load8(st, raw, inb, cache);
+    return blend(st, raw, inb, require_all, out);
   }
 
   // blend_color.
@@ -310,8 +352,10 @@ struct Sampler {
   // register use of the raycaster, where the normal's peak would otherwise
   // set the register allocation, and so the occupancy, of the whole march.
   __device__ bool normal(V3 p, bool require_all, V3& out) const {
-    float gr[3];
-    constexpr int kBatch = 2;
+    // the six samples kBatch at a time: a direct store reads two at once; a
+    // hashed store's slot probes take too many registers for two
+    constexpr int kBatch = Store::kDirect ? 2 : 1;
+    float v[6];
 #pragma unroll
     for (int a0 = 0; a0 < 6; a0 += kBatch) {
       Stencil st[kBatch];
@@ -326,14 +370,10 @@ struct Sampler {
         load8(st[i], raw[i], inb[i], cache);
       }
 #pragma unroll
-      for (int i = 0; i < kBatch; i += 2) {
-        float plus, minus;
-        if (!blend(st[i], raw[i], inb[i], require_all, plus)) return false;
-        if (!blend(st[i + 1], raw[i + 1], inb[i + 1], require_all, minus)) return false;
-        gr[(a0 + i) >> 1] = plus - minus;
-      }
+      for (int i = 0; i < kBatch; ++i)
+        if (!blend(st[i], raw[i], inb[i], require_all, v[a0 + i])) return false;
     }
-    const V3 dir = mk3(gr[0], gr[1], gr[2]);
+    const V3 dir = mk3(v[0] - v[1], v[2] - v[3], v[4] - v[5]);
     const float len = norm3(dir);
     if (len <= 0.0F) return false;
     out = div3(dir, len);

@@ -276,6 +276,18 @@ def test_2048_hashed_hall_bit_exact(K, ko, torch_gpu):
     rkeys, rvox, _ = vol.blocks()
     assert np.array_equal(keys, rkeys) and len(keys) > 10_000
     assert np.array_equal(vox.view(np.uint32), rvox.view(np.uint32))
+    # the raycast over metres of unobserved space (whole 32^3-voxel cells of
+    # skip blocks, blocks without observed voxels next to observed ones)
+    for i, level in ((135, 0), (135, 2), (290, 0)):
+        pose = scene.pose(i)
+        maps = p.raycast(pose, level)
+        pts, nrm, col, _ = ko.raycast(vol, pose, level_intr(level), W >> level, H >> level)
+        gp, gn = host(maps.points)[..., :3], host(maps.normals)[..., :3]
+        print(f"raycast pose {i} level {level}: {np.isfinite(pts).all(-1).mean():.3f} of the rays hit", flush=True)
+        assert np.isfinite(pts).all(-1).mean() > 0.9
+        assert same_bits(gp, pts), first_mismatch(gp, pts)
+        assert same_bits(gn, nrm), first_mismatch(gn, nrm)
+        assert np.array_equal(host(maps.colors).view(np.uint32), col)
     gpu = p.extract_mesh(0.0)
     pos, nrm, col, tri = ko.mesh(vol, 0.0)
     print(f"hall mesh: {len(pos)} vertices, {len(tri)} triangles")
