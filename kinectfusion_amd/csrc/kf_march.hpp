@@ -6,6 +6,7 @@
 #pragma once
 
 #include "kf_sampler.hpp"
+#include "kf_steps.hpp"
 
 namespace kf {
 
@@ -121,8 +122,9 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
     uint32_t cur_nbr = 0U;  // observed blocks among the current block's 3^3 (hashed store)
     SlotCache slots;  // the last block a sample probed (hashed store)
     // The samples of a skip or free block that lie deeper than kSafe voxels
-    // inside it on every axis are stepped in a tight loop (a float add and a
-    // compare each): [enter, exit) is that stretch of ray lengths, from the
+    // inside it on every axis are stepped in one run (the float additions of
+    // the reference's loop, bit for bit): [enter, exit) is that stretch of
+    // ray lengths, from the
     // ray in grid coordinates, go + gd * len.  The grid coordinates the
     // march computes for a sample differ from the exact ones by a few float
     // ulps of the magnitudes involved (< 4e-3 voxel for points within 2^14
@@ -145,6 +147,24 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
     // safe interior at once, so a ray crossing unobserved space reads one
     // class per 32^3-voxel cell instead of one per block.
     constexpr float kSafe = 0.05F;
+    // len <= max_depth is len < depth_lim: a run of steps lasts while len < min(exit, depth_lim)
+    const float depth_lim = nextafterf(rp.max_depth, __builtin_inff());
+    float unused_last;
+    // A run of steps: a direct store's runs are short and a plain loop is
+    // faster there (the closed form cost 1.5 % of the headline frame rate);
+    // a hashed store's rays cross long unobserved runs, where advance_to()
+    // gained 5.6 % in the 2048^3 hall.
+    auto run_to = [](float l, float step, float lim, float& last) {
+      if constexpr (Store::kDirect) {
+        while (l < lim) {
+          last = l;
+          l += step;
+        }
+        return l;
+      } else {
+        return advance_to(l, step, lim, last);
+      }
+    };
     // Empty blocks pay where a sample costs a hash probe before its corner
     // reads; a direct store reads the corners at once, and the extra class
     // word per block was slower there (-1.4 % headline frame rate).
@@ -191,8 +211,7 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
         if (cur_skip) {  // no corner is observed: no TSDF
           have_prev = false;
           len += base_step;
-          if (len >= enter)
-            while (len < exit && len <= rp.max_depth) len += base_step;
+          if (len >= enter) len = run_to(len, base_step, fminf(exit, depth_lim), unused_last);
           continue;
         }
         if (cur_free) {  // every corner reads exactly 1: no read, free steps
@@ -201,18 +220,14 @@ __device__ inline void raycast_pixel(const Sampler<Kind, Store>& vol, const RayP
           prev_t = 1.0F;
           float last = len;
           len += free_step;
-          if (len >= enter)
-            while (len < exit && len <= rp.max_depth) {
-              last = len;
-              len += free_step;
-            }
+          if (len >= enter) len = run_to(len, free_step, fminf(exit, depth_lim), last);
           prev_p = add3(pose.t, scale3(dir, last));  // the last free sample's point
           continue;
         }
         if (cur_empty && len >= enter && len < exit) {  // a stencil inside an empty block: no TSDF
           have_prev = false;
           len += base_step;
-          while (len < exit && len <= rp.max_depth) len += base_step;
+          len = run_to(len, base_step, fminf(exit, depth_lim), unused_last);
           continue;
         }
         if (cur_empty) {
