@@ -919,34 +919,65 @@ __global__ void __launch_bounds__(256) band_walk_kernel(Grid g, FrameParams f, c
   }
 }
 
+// Exclusive prefix sum of v over the 256 threads of a workgroup, and the
+// workgroup's total (`sums`: 4 words of LDS).
+__device__ inline uint32_t workgroup_scan(uint32_t v, uint32_t* sums, uint32_t& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;  // inclusive scan within the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  __syncthreads();  // the words may still hold an earlier scan's totals
+  if (lane == 63) sums[wave] = x;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wave; ++w) before += sums[w];
+  total = sums[0] + sums[1] + sums[2] + sums[3];
+  return before + x - v;
+}
+
 // Pool slots for this frame's new blocks and the frame's work list (each
-// visited block once).  Appends are aggregated per wave.
+// visited block once).  A workgroup scans kAssignRun consecutive buckets
+// per thread and takes its share of the slot and list counters with one
+// atomic each: with one atomic per wave, the waves' appends to the same two
+// words serialised (171 us per frame for 2^20 buckets in the 2048^3 hall).
+constexpr int kAssignRun = 16;
 __global__ void __launch_bounds__(256) assign_slots_kernel(HashView hv, const uint32_t* __restrict__ stamp,
                                                             uint32_t* __restrict__ sp,
                                                             uint32_t* __restrict__ block_of_slot,
                                                             uint32_t* __restrict__ work, uint32_t capacity,
                                                             uint32_t buckets) {
+  __shared__ uint32_t sums[4];
+  __shared__ uint32_t base;
   const uint32_t frame = sp[3];
-  const int lane = threadIdx.x & 63;
-  const uint32_t stride = gridDim.x * 256U;
-  const uint32_t rounds = (buckets + stride - 1) / stride;
-  for (uint32_t r = 0; r < rounds; ++r) {
-    const uint32_t b = r * stride + blockIdx.x * 256U + threadIdx.x;
-    unsigned long long e = kHashEmpty;
-    bool visited = false;
+  const uint32_t first = blockIdx.x * (256U * kAssignRun) + threadIdx.x;
+  unsigned long long e[kAssignRun];
+  uint32_t visited = 0U, fresh = 0U;  // bit r: bucket first + 256 r
+#pragma unroll
+  for (int r = 0; r < kAssignRun; ++r) {
+    const uint32_t b = first + 256U * r;
+    e[r] = kHashEmpty;
     if (b < buckets) {
-      e = hv.table[b];
-      visited = e != kHashEmpty && stamp[b] == frame;
+      e[r] = hv.table[b];
+      if (e[r] != kHashEmpty && stamp[b] == frame) {
+        visited |= 1U << r;
+        if (static_cast<uint32_t>(e[r]) == kPendingSlot) fresh |= 1U << r;
+      }
     }
-    uint32_t slot = visited ? static_cast<uint32_t>(e) : kNoSlot;
-    const bool fresh = visited && slot == kPendingSlot;
-    const unsigned long long fm = __ballot(fresh);
-    uint32_t base = 0;
-    if (lane == 0 && fm != 0ULL) base = atomicAdd(&sp[0], static_cast<uint32_t>(__popcll(fm)));
-    base = __shfl(base, 0);
-    if (fresh) {
-      const uint32_t s = base + static_cast<uint32_t>(__popcll(fm & ((1ULL << lane) - 1ULL)));
-      const uint32_t key = static_cast<uint32_t>(e >> 32);
+  }
+  uint32_t total;
+  uint32_t s = workgroup_scan(static_cast<uint32_t>(__popc(fresh)), sums, total);
+  if (threadIdx.x == 0) base = total != 0U ? atomicAdd(&sp[0], total) : 0U;
+  __syncthreads();
+  s += base;
+  uint32_t listed = 0U;  // bit r: the bucket's block goes to the work list
+#pragma unroll
+  for (int r = 0; r < kAssignRun; ++r) {
+    uint32_t slot = static_cast<uint32_t>(e[r]);
+    if (((fresh >> r) & 1U) != 0U) {
+      const uint32_t key = static_cast<uint32_t>(e[r] >> 32);
       if (s < capacity) {
         slot = s;
         block_of_slot[s] = key;
@@ -954,15 +985,20 @@ __global__ void __launch_bounds__(256) assign_slots_kernel(HashView hv, const ui
         slot = kNoSlot;
         atomicAdd(&sp[4], 1U);
       }
-      hv.table[b] = (static_cast<unsigned long long>(key) << 32) | slot;
+      ++s;
+      hv.table[first + 256U * r] = (static_cast<unsigned long long>(key) << 32) | slot;
+      e[r] = (static_cast<unsigned long long>(key) << 32) | slot;
     }
-    const bool listed = visited && slot != kNoSlot;
-    const unsigned long long lm = __ballot(listed);
-    uint32_t wbase = 0;
-    if (lane == 0 && lm != 0ULL) wbase = atomicAdd(&sp[1], static_cast<uint32_t>(__popcll(lm)));
-    wbase = __shfl(wbase, 0);
-    if (listed) work[wbase + __popcll(lm & ((1ULL << lane) - 1ULL))] = slot;
+    if (((visited >> r) & 1U) != 0U && slot != kNoSlot) listed |= 1U << r;
   }
+  uint32_t w = workgroup_scan(static_cast<uint32_t>(__popc(listed)), sums, total);
+  __syncthreads();  // every thread has read `base`
+  if (threadIdx.x == 0) base = total != 0U ? atomicAdd(&sp[1], total) : 0U;
+  __syncthreads();
+  w += base;
+#pragma unroll
+  for (int r = 0; r < kAssignRun; ++r)
+    if (((listed >> r) & 1U) != 0U) work[w++] = static_cast<uint32_t>(e[r]);
 }
 
 // Every voxel of the listed blocks (BlockVoxels, clipped to the grid): one
@@ -1326,7 +1362,9 @@ void launch_integrate(kf_context* ctx, const kf_depth_frame& fr, bool count, boo
     hipLaunchKernelGGL(band_walk_kernel, dim3(tiles_x, tiles_y), dim3(kTile, kTile), 0, ctx->stream, ctx->grid, f,
                        fr.depth, hash_view(ctx), ctx->hash_stamp, ctx->sparse, limit,
                        static_cast<uint32_t>(ctx->capacity), step);
-    hipLaunchKernelGGL(assign_slots_kernel, dim3(1024), dim3(256), 0, ctx->stream, hash_view(ctx), ctx->hash_stamp,
+    const uint32_t per_group = 256U * kAssignRun;
+    hipLaunchKernelGGL(assign_slots_kernel, dim3((static_cast<uint32_t>(ctx->buckets) + per_group - 1) / per_group),
+                       dim3(256), 0, ctx->stream, hash_view(ctx), ctx->hash_stamp,
                        ctx->sparse, ctx->block_of_slot, ctx->work, static_cast<uint32_t>(ctx->capacity),
                        static_cast<uint32_t>(ctx->buckets));
     if (timing) KF_HIP(hipEventRecord(ctx->ev[12], ctx->stream));
