@@ -1002,10 +1002,14 @@ __global__ void __launch_bounds__(256) assign_slots_kernel(HashView hv, const ui
 }
 
 // Every voxel of the listed blocks (BlockVoxels, clipped to the grid): one
-// workgroup per block, x fastest, so voxel accesses are contiguous.  Hashed
-// (Dense = false): the list holds pool slots, block_of_slot their keys, and
-// a block's voxels are its pool slot's 512.  Dense band: the list holds
-// block keys and voxels sit at their dense indices.  n: the list's length.
+// wave per block, each lane the z-column of one (x, y) of it (two runs of 4
+// voxels with the column's shared transform), so for every z the wave's
+// voxels are contiguous.  The block's flags are wave ballots: no workgroup
+// barrier (one workgroup per block with three barriers per block was
+// latency-bound at 100 us per hall frame).  Hashed (Dense = false): the list
+// holds pool slots, block_of_slot their keys, and a block's voxels are its
+// pool slot's 512.  Dense band: the list holds block keys and voxels sit at
+// their dense indices.  n: the list's length.
 template <int Kind, bool Color, bool Count, bool Dense>
 __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FrameParams f, const float4* __restrict__ pix,
                                                                 const uint32_t* __restrict__ color_img,
@@ -1017,59 +1021,58 @@ __global__ void __launch_bounds__(256) integrate_blocks_kernel(Grid g, FramePara
                                                                 unsigned long long* __restrict__ fused,
                                                                 uint32_t* __restrict__ obs_bits, BlockMap bm,
                                                                 uint32_t* __restrict__ part_bits) {
-  __shared__ int any_fused;
-  __shared__ int all_ok, all_complete;
   Rigid w2c;
   if (!block_w2c(f, w2c)) return;
   const uint32_t n = *count;
+  const int lane = static_cast<int>(threadIdx.x & 63);
+  const uint32_t waves = gridDim.x * 4U;
   unsigned int my_fused = 0, my_color = 0;
-  for (uint32_t li = blockIdx.x; li < n; li += gridDim.x) {
+  for (uint32_t li = blockIdx.x * 4U + (threadIdx.x >> 6); li < n; li += waves) {
     const uint32_t slot = work[li];
     const uint32_t key = Dense ? slot : block_of_slot[slot];
     const int bx = static_cast<int>(key % static_cast<uint32_t>(bm.bx));
     const int by = static_cast<int>((key / static_cast<uint32_t>(bm.bx)) % static_cast<uint32_t>(bm.by));
     const int bz = static_cast<int>(key / static_cast<uint32_t>(bm.bx * bm.by));
-    if (threadIdx.x == 0) {
-      any_fused = 0;
-      all_ok = 1;
-      all_complete = 1;
-    }
-    __syncthreads();
-    int xs[2], ys[2], zs[2];
-    size_t vis[2];
-    bool inside[2];
-#pragma unroll
-    for (int pass = 0; pass < 2; ++pass) {
-      const int i = static_cast<int>(threadIdx.x) + pass * 256;
-      xs[pass] = bx * 8 + (i & 7);
-      ys[pass] = by * 8 + ((i >> 3) & 7);
-      zs[pass] = bz * 8 + (i >> 6);
-      inside[pass] = xs[pass] < g.nx && ys[pass] < g.ny && zs[pass] < g.nz;
-      if constexpr (Dense) vis[pass] = inside[pass] ? g.flat(xs[pass], ys[pass], zs[pass]) : 0;
-      else vis[pass] = static_cast<size_t>(slot) * kBlockVoxels + i;
-    }
+    const int x = bx * 8 + (lane & 7), y = by * 8 + (lane >> 3);
     bool any = false, ones_ok = true, complete = true;
-    fuse_batch<Kind, Color, Count, 2>(g, f, w2c, pix, color_img, pool, cpool, xs, ys, zs, vis, inside, my_fused,
-                                      my_color, any, ones_ok, complete);
-    if (any) any_fused = 1;
-    if (!ones_ok) all_ok = 0;
-    if (!complete) all_complete = 0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int fb = bm.flat(bx, by, bz);
-      if (any_fused != 0) atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
-      const uint32_t nib = 0xFU << ((fb & 7) * 4);
-      if (all_ok == 0) atomicAnd(&part_bits[fb >> 3], ~nib);
-      else if (all_complete != 0) atomicOr(&part_bits[fb >> 3], nib);  // else unchanged
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      int xs[4], ys[4], zs[4];
+      size_t vis[4];
+      bool inside[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int zi = half * 4 + k;
+        xs[k] = x;
+        ys[k] = y;
+        zs[k] = bz * 8 + zi;
+        inside[k] = x < g.nx && y < g.ny && zs[k] < g.nz;
+        if constexpr (Dense) vis[k] = inside[k] ? g.flat(x, y, zs[k]) : 0;
+        else vis[k] = static_cast<size_t>(slot) * kBlockVoxels + static_cast<size_t>(zi * 64 + lane);
+      }
+      bool a = false, o = true, c = true;
+      fuse_batch<Kind, Color, Count, 4, true>(g, f, w2c, pix, color_img, pool, cpool, xs, ys, zs, vis, inside,
+                                              my_fused, my_color, a, o, c);
+      any = any || a;
+      ones_ok = ones_ok && o;
+      complete = complete && c;
     }
-    __syncthreads();
+    const bool any_fused = __ballot(any) != 0ULL, all_ok = __ballot(!ones_ok) == 0ULL,
+               all_complete = __ballot(!complete) == 0ULL;
+    if (lane == 0) {
+      const int fb = bm.flat(bx, by, bz);
+      if (any_fused) atomicOr(&obs_bits[fb >> 5], 1U << (fb & 31));
+      const uint32_t nib = 0xFU << ((fb & 7) * 4);
+      if (!all_ok) atomicAnd(&part_bits[fb >> 3], ~nib);
+      else if (all_complete) atomicOr(&part_bits[fb >> 3], nib);  // else unchanged
+    }
   }
   if (Count) {
     for (int off = 32; off > 0; off >>= 1) {
       my_fused += __shfl_xor(my_fused, off);
       my_color += __shfl_xor(my_color, off);
     }
-    if ((threadIdx.x & 63) == 0) {
+    if (lane == 0) {
       atomicAdd(&fused[0], static_cast<unsigned long long>(my_fused));
       atomicAdd(&fused[1], static_cast<unsigned long long>(my_color));
     }
