@@ -13,7 +13,7 @@
 // 0.03, 0.03) is the twist (omega rad, t m) of the motion from the previous
 // camera to the current one: T_cur = T_prev * [R(omega) | t].
 //
-// Every layer is one GEMM on the matrix cores (v_mfma_f32_16x16x16_bf16,
+// Every layer is one GEMM on the matrix cores (v_mfma_f32_16x16x32_bf16,
 // FP32 accumulation): convolutions as im2col (K = 9 * Cin, k = (ky * 3 + kx)
 // * Cin + ci) times W^T, their weight gradients as split-K GEMMs over the
 // batch-pixel axis with fixed-order partial sums, input gradients as GEMMs
@@ -30,7 +30,8 @@ namespace {
 
 using kf::fail;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kInH = 120, kInW = 160;
@@ -90,7 +91,7 @@ std::vector<Shape> network_shapes(size_t* total) {
 // ---------------------------------------------------------------------------
 // GEMM: C(m, n) = sum_k A(m, k) B(k, n), bf16 operands with arbitrary
 // strides, FP32 accumulation.  64x64 tile per 256-thread workgroup, each of
-// the four waves a 32x32 quadrant of 2x2 16x16x16 MFMA blocks, K staged 32
+// the four waves a 32x32 quadrant of 2x2 16x16x32 MFMA blocks, K staged 32
 // at a time through LDS.  gridDim.z splits K: every split writes its FP32
 // partial tile and a second kernel adds the partials in split order 0, 1,
 // ... and applies the epilogue, so the sum does not depend on timing.
@@ -126,7 +127,7 @@ struct Gemm {
   int scale_out;  // outf receives v * out_scale(n) (the twist)
 };
 
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 4;
+constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;  // 80-byte rows: 16-byte fragment reads stay aligned
 constexpr int kStageRows = 256 / BK;           // rows one pass of the workgroup stages
 constexpr int kRowsPerThread = BM / kStageRows;  // 8
 constexpr int kMaxTiles = 256;                   // at most this many tiles split K
@@ -207,19 +208,19 @@ __global__ void __launch_bounds__(256) gemm_kernel(Gemm g) {
       Bs[r][sc] = (n < g.N && kin) ? g.b[k * g.sbk + n * g.sbn] : static_cast<uint16_t>(0);
     }
     __syncthreads();
-#pragma unroll
-    for (int ks = 0; ks < BK; ks += 16) {
-      bf16x4 av[2], bv[2];
+    {  // the staged 32 k as one 16x16x32 step
+      s16x8 av[2], bv[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        av[i] = *reinterpret_cast<const bf16x4*>(&As[wm + i * 16 + li][ks + lq * 4]);
-        bv[i] = *reinterpret_cast<const bf16x4*>(&Bs[wn + i * 16 + li][ks + lq * 4]);
+        av[i] = *reinterpret_cast<const s16x8*>(&As[wm + i * 16 + li][lq * 8]);
+        bv[i] = *reinterpret_cast<const s16x8*>(&Bs[wn + i * 16 + li][lq * 8]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, av[i]),
+                                                              __builtin_bit_cast(bf16x8, bv[j]), acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
@@ -264,14 +265,15 @@ __global__ void gemm_finish_kernel(Gemm g, int splits) {
 // ---------------------------------------------------------------------------
 // Small GEMM (few output tiles, K up to a few thousand: the layers of a pose
 // proposal, and the forward layers of small batches).  One workgroup per
-// 16 x 16 output tile; its Waves waves (8, or 16 for long reductions) split the K steps (16 wide)
+// 16 x 16 output tile; its Waves waves (8, or 16 for long reductions) split
+// the K steps (32 wide: v_mfma_f32_16x16x32_bf16, gfx950's bf16 shape)
 // between them and load their operand fragments straight from global memory
 // into registers, kSmallChunk steps at a time, so that a layer costs one or
 // two memory round trips instead of one per K step.  Lane l holds
-// A[l % 16][4 (l / 16) + j] and B[4 (l / 16) + j][l % 16] (j = 0..3) of a
+// A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16] (j = 0..7) of a
 // step and D[4 (l / 16) + r][l % 16] of the tile.  The per-wave tiles are
 // added in wave order through LDS, so the result does not depend on timing.
-constexpr int kSmallChunk = 10;
+constexpr int kSmallChunk = 5;
 
 // A(m, k) of the im2col matrix for a row whose receptive-field origin is
 // (oy, ox) of pair ob (oy < -2 marks a row past M).
@@ -305,56 +307,57 @@ __global__ void __launch_bounds__(Waves * 64) gemm_small_kernel(Gemm g, int vec_
     ox = 2 * xo - 1;
     ob = t / g.conv.Ho;
   }
-  const int steps = (g.K + 15) / 16;
+  const int steps = (g.K + 31) / 32;
   const int per = (steps + Waves - 1) / Waves;
   const int s_begin = wave * per, s_end = min(steps, s_begin + per);
   f32x4 acc = f32x4{0.0F, 0.0F, 0.0F, 0.0F};
   for (int c0 = s_begin; c0 < s_end; c0 += kSmallChunk) {
-    bf16x4 af[kSmallChunk], bfr[kSmallChunk];
+    s16x8 af[kSmallChunk], bfr[kSmallChunk];
 #pragma unroll
     for (int c = 0; c < kSmallChunk; ++c) {
-      af[c] = bf16x4{0, 0, 0, 0};
-      bfr[c] = bf16x4{0, 0, 0, 0};
-      const int k = (c0 + c) * 16 + lq * 4;
+      af[c] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      bfr[c] = s16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      const int k = (c0 + c) * 32 + lq * 8;
       if (c0 + c >= s_end || k >= g.K) continue;
-      const bool full = k + 3 < g.K;
+      const bool full = k + 7 < g.K;
       if (mok) {
         if (implicit) {
           if (vec_a != 0 && full) {
-            // C % 4 == 0: the four k are channels ch..ch+3 of one tap
+            // C % 8 == 0: the eight k are channels ch..ch+7 of one tap
             const int tap = k / g.conv.C, ch = k - tap * g.conv.C;
             const int ky = tap / 3, kx = tap - ky * 3;
             const int yi = oy + ky, xi = ox + kx;
             if (yi >= 0 && yi < g.conv.H && xi >= 0 && xi < g.conv.W) {
-              af[c] = *reinterpret_cast<const bf16x4*>(
+              af[c] = *reinterpret_cast<const s16x8*>(
                   g.conv.x + ((static_cast<long long>(ob) * g.conv.H + yi) * g.conv.W + xi) * g.conv.C + ch);
             }
           } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < 8; ++j)
               if (k + j < g.K) af[c][j] = as_short(conv_a(g.conv, oy, ox, ob, k + j));
           }
         } else if (vec_a != 0 && full) {
-          af[c] = *reinterpret_cast<const bf16x4*>(g.a + m * g.sam + k);
+          af[c] = *reinterpret_cast<const s16x8*>(g.a + m * g.sam + k);
         } else {
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+          for (int j = 0; j < 8; ++j)
             if (k + j < g.K) af[c][j] = as_short(g.a[m * g.sam + (k + j) * g.sak]);
         }
       }
       if (nok) {
         if (vec_b != 0 && full) {
-          bfr[c] = *reinterpret_cast<const bf16x4*>(g.b + n * g.sbn + k);
+          bfr[c] = *reinterpret_cast<const s16x8*>(g.b + n * g.sbn + k);
         } else {
 #pragma unroll
-          for (int j = 0; j < 4; ++j)
+          for (int j = 0; j < 8; ++j)
             if (k + j < g.K) bfr[c][j] = as_short(g.b[(k + j) * g.sbk + n * g.sbn]);
         }
       }
     }
 #pragma unroll
     for (int c = 0; c < kSmallChunk; ++c)
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af[c], bfr[c], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, af[c]), __builtin_bit_cast(bf16x8, bfr[c]),
+                                                    acc, 0, 0, 0);
   }
 #pragma unroll
   for (int r = 0; r < 4; ++r) red[wave][lq * 4 + r][li] = acc[r];
@@ -563,19 +566,19 @@ void gemm(kf_posenet* net, hipStream_t s, Gemm g) {
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   const int tiles = tm * tn;
   if (tiles <= 128 && g.K <= 4096) {
-    // vector fragment loads need four consecutive k in memory, 8-byte aligned
-    auto aligned8 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 7U) == 0U; };
+    // vector fragment loads need eight consecutive k in memory, 16-byte aligned
+    auto aligned16 = [](const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15U) == 0U; };
     int vec_a = 0;
     if (g.a == nullptr) {
-      vec_a = g.conv.raw_prev == nullptr && g.conv.C % 4 == 0 && aligned8(g.conv.x) ? 1 : 0;
+      vec_a = g.conv.raw_prev == nullptr && g.conv.C % 8 == 0 && aligned16(g.conv.x) ? 1 : 0;
     } else {
-      vec_a = g.sak == 1 && g.sam % 4 == 0 && aligned8(g.a) ? 1 : 0;
+      vec_a = g.sak == 1 && g.sam % 8 == 0 && aligned16(g.a) ? 1 : 0;
     }
-    const int vec_b = g.sbk == 1 && g.sbn % 4 == 0 && aligned8(g.b) ? 1 : 0;
+    const int vec_b = g.sbk == 1 && g.sbn % 8 == 0 && aligned16(g.b) ? 1 : 0;
     g.partial = nullptr;
     // a reduction longer than 8 waves x kSmallChunk steps (fc1: K = 2560) takes 16 waves: one read round
     const dim3 grid((g.M + 15) / 16, (g.N + 15) / 16);
-    if ((g.K + 15) / 16 > 8 * kSmallChunk) {
+    if ((g.K + 31) / 32 > 8 * kSmallChunk) {
       hipLaunchKernelGGL(gemm_small_kernel<16>, grid, dim3(16 * 64), 0, s, g, vec_a, vec_b);
     } else {
       hipLaunchKernelGGL(gemm_small_kernel<8>, grid, dim3(8 * 64), 0, s, g, vec_a, vec_b);
