@@ -54,13 +54,16 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_forward_matches_oracle(PN, P, torch_gpu):
+@pytest.mark.parametrize("batch", [4, 1])
+def test_forward_matches_oracle(PN, P, torch_gpu, batch):
+    """A batch, and one pair (the pipeline's proposal: every layer a
+    one-workgroup-per-tile GEMM)."""
     net = PN.PoseNet(PN.PoseNetConfig(batch_capacity=8, seed=3))
-    raw, dev, _, _ = random_pairs(torch_gpu, 4)
+    raw, dev, _, _ = random_pairs(torch_gpu, batch)
     out = net.forward(dev).cpu().numpy()
     ref, _ = P.forward(net.parameters().astype(np.float64), P.inputs_from_pairs(raw), emulate_bf16=True)
     err = rel(out, ref * P.OUT_SCALE)
-    print(f"forward relative error {err:.2e}")
+    print(f"batch {batch}: forward relative error {err:.2e}")
     assert err < FWD_TOL
     net.close()
 
