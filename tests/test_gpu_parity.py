@@ -473,24 +473,33 @@ def test_captured_frames_match_eager_frames(K, scene, torch_gpu):
     """Tracking frames replay one captured HIP graph; with stage timing on,
     every frame is enqueued eagerly instead.  Both give byte-identical poses,
     voxels and colours, also across an ICP-option change (which re-captures)
-    and a frame without colour."""
+    and a frame without colour, with the frames enqueued back to back, and
+    the pyramid views after the last frame are that frame's in both."""
     def run(stage_timing):
         p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL, stage_timing=stage_timing))
         loose = K.ProjectiveIcpOptions(max_point_distance=0.2)
-        for i in range(6):
+        frames = []
+        for i in range(9):
             depth, rgba, _ = scene.frame(i)
-            opts = loose if i == 3 else None
-            color = None if i == 4 else to_dev(torch_gpu, rgba)
-            p.process_frame(to_dev(torch_gpu, depth), color, icp=opts, report=False)
+            frames.append((to_dev(torch_gpu, depth), None if i == 4 else to_dev(torch_gpu, rgba)))
+        torch_gpu.cuda.synchronize()
+        for i, (depth, color) in enumerate(frames):
+            p.process_frame(depth, color, icp=loose if i == 3 else None, report=False, sync_inputs=False)
         p.synchronize()
         vox, col = p.host_view()
-        out = (p.trajectory(), vox.copy(), col.copy())
+        src = K.PyramidSource(p)
+        levels = [{k: v.cpu().numpy().copy() for k, v in src.level(l).items() if k != "intrinsics"}
+                  for l in range(3)]
+        out = (p.trajectory(), vox.copy(), col.copy(), levels)
         p.close()
         return out
 
     eager, graph = run(True), run(False)
     assert np.array_equal(eager[0], graph[0])
     assert np.array_equal(eager[1], graph[1]) and np.array_equal(eager[2], graph[2])
+    for a, b in zip(eager[3], graph[3]):
+        for k in a:
+            assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
 
 
 @pytest.mark.parametrize("res", [256, 200])
