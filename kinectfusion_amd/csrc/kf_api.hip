@@ -326,8 +326,9 @@ void fetch_state_checked(kf_context* ctx) {
   report_sync_timeouts(ctx);
 }
 
-void build_pyramid(kf_context* ctx, const uint16_t* depth) {
-  launch_bilateral(ctx, depth);
+// filtered: level 0's depth already holds the filtered frame
+void build_pyramid(kf_context* ctx, const uint16_t* depth, bool filtered = false) {
+  if (!filtered) launch_bilateral(ctx, depth);
   if (!launch_pyramid3(ctx))
     for (int l = 0; l < ctx->cfg.levels; ++l) launch_level_maps(ctx, l);
   ctx->levels_built = ctx->cfg.levels;
@@ -392,10 +393,10 @@ void drop_graph(kf_context* ctx) {
 // coarse-to-fine tracking (not on the first frame), integration, and the
 // trajectory record.  No call here waits on the device.
 void enqueue_frame(kf_context* ctx, const uint16_t* depth, const uint32_t* color, const kf_icp_options& opt,
-                   bool first) {
+                   bool first, bool filtered = false) {
   const bool timing = ctx->cfg.stage_timing != 0;
   record(ctx, 0);
-  build_pyramid(ctx, depth);
+  build_pyramid(ctx, depth, filtered);
   record(ctx, 1);
   const kf_depth_frame frame = frame_from_pyramid(ctx, depth, color);
   if (first) {
@@ -903,18 +904,25 @@ kf_status kf_process_frame(kf_context* ctx, const uint16_t* depth, const uint32_
     if (eager) {
       enqueue_frame(ctx, depth, color, *opt, ctx->frames == 0);
     } else {
-      // tracking frames replay one captured graph over staging copies of the inputs
-      const size_t npx = static_cast<size_t>(ctx->cfg.width) * ctx->cfg.height;
-      hipLaunchKernelGGL(stage_inputs_kernel, dim3(static_cast<unsigned>((npx + 1023) / 1024)), dim3(256), 0,
-                         ctx->stream, depth, ctx->raw_depth, color, ctx->raw_color, npx);
-      KF_HIP(hipGetLastError());
+      // tracking frames replay one captured graph over staging copies of the
+      // inputs; the bilateral filter, launched before it, makes the copies
+      // in its own pass
       const bool has_color = color != nullptr;
+      const bool filtered = bilateral_active(ctx->cfg);
+      if (filtered) {
+        launch_bilateral(ctx, depth, StageCopy{ctx->raw_depth, color, ctx->raw_color});
+      } else {
+        const size_t npx = static_cast<size_t>(ctx->cfg.width) * ctx->cfg.height;
+        hipLaunchKernelGGL(stage_inputs_kernel, dim3(static_cast<unsigned>((npx + 1023) / 1024)), dim3(256), 0,
+                           ctx->stream, depth, ctx->raw_depth, color, ctx->raw_color, npx);
+        KF_HIP(hipGetLastError());
+      }
       if (ctx->graph_exec == nullptr || has_color != ctx->graph_color ||
           std::memcmp(&ctx->graph_opt, opt, sizeof(kf_icp_options)) != 0) {
         drop_graph(ctx);
         KF_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         try {
-          enqueue_frame(ctx, ctx->raw_depth, has_color ? ctx->raw_color : nullptr, *opt, false);
+          enqueue_frame(ctx, ctx->raw_depth, has_color ? ctx->raw_color : nullptr, *opt, false, filtered);
         } catch (...) {
           hipGraph_t broken = nullptr;
           (void)hipStreamEndCapture(ctx->stream, &broken);

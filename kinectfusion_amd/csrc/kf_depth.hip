@@ -29,11 +29,14 @@ constexpr int kMaxRadius = 8;
 // reference's row-major order, an unusable tap (outside the image or without
 // depth, both staged as -1) adding +0, which leaves them unchanged (they are
 // never -0).  R = 0: the radius is a run-time value.
+//
+// stage (optional): each output pixel's raw depth and colour are also copied
+// into the staging buffers a captured frame reads, in the same pass.
 template <int R>
 __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restrict__ in,
                                                          uint16_t* __restrict__ out, int w, int h,
                                                          float scale, float lo, float hi, int radius_arg,
-                                                         float ss, float sr) {
+                                                         float ss, float sr, StageCopy stage) {
   const int radius = R > 0 ? R : radius_arg;
   __shared__ float meters[(kTy + 2 * kMaxRadius) * (kTx + 2 * kMaxRadius)];
   __shared__ float raws[(kTy + 2 * kMaxRadius) * (kTx + 2 * kMaxRadius)];
@@ -60,6 +63,11 @@ __global__ void __launch_bounds__(256) bilateral_kernel(const uint16_t* __restri
   __syncthreads();
   const int col = blockIdx.x * kTx + threadIdx.x, row = blockIdx.y * kTy + threadIdx.y;
   if (col >= w || row >= h) return;
+  if (stage.depth != nullptr) {
+    const size_t i = static_cast<size_t>(row) * w + col;
+    stage.depth[i] = in[i];
+    if (stage.color_in != nullptr) stage.color[i] = stage.color_in[i];
+  }
   const int cl = (threadIdx.y + radius) * tw + threadIdx.x + radius;
   const float cm = meters[cl];
   uint16_t result = 0;
@@ -321,10 +329,13 @@ __global__ void __launch_bounds__(256) pyramid3_kernel(const uint16_t* __restric
 
 }  // namespace
 
-void launch_bilateral(kf_context* ctx, const uint16_t* raw) {
+bool bilateral_active(const kf_config& c) { return c.bilateral_filter != 0 && c.bilateral_radius != 0; }
+
+void launch_bilateral(kf_context* ctx, const uint16_t* raw, StageCopy stage) {
   const kf_config& c = ctx->cfg;
   LevelBuffers& L = ctx->lv[0];
-  if (!c.bilateral_filter || c.bilateral_radius == 0) {
+  if (!bilateral_active(c)) {
+    if (stage.depth != nullptr) fail(KF_ERR_STATE, "launch_bilateral: staging needs the filter pass");
     KF_HIP(hipMemcpyAsync(L.depth, raw, sizeof(uint16_t) * L.w * L.h, hipMemcpyDeviceToDevice,
                           ctx->stream));
     return;
@@ -334,10 +345,10 @@ void launch_bilateral(kf_context* ctx, const uint16_t* raw) {
   const float sr = -0.5F / (c.bilateral_depth_sigma * c.bilateral_depth_sigma);
   if (c.bilateral_radius == 3)
     hipLaunchKernelGGL(bilateral_kernel<3>, grid, block, 0, ctx->stream, raw, L.depth, L.w, L.h,
-                     c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr);
+                     c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr, stage);
   else
     hipLaunchKernelGGL(bilateral_kernel<0>, grid, block, 0, ctx->stream, raw, L.depth, L.w, L.h,
-                     c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr);
+                     c.depth_scale, c.min_depth, c.max_depth, c.bilateral_radius, ss, sr, stage);
   KF_HIP(hipGetLastError());
 }
 

@@ -512,7 +512,14 @@ void hip_check(hipError_t err, const char* what);
 #define KF_HIP(expr) ::kf::hip_check((expr), #expr)
 
 // Launch helpers implemented per stage.
-void launch_bilateral(kf_context* ctx, const uint16_t* raw);
+// Staging copies of a frame's raw depth and colour (colour optional).
+struct StageCopy {
+  uint16_t* depth = nullptr;
+  const uint32_t* color_in = nullptr;
+  uint32_t* color = nullptr;
+};
+bool bilateral_active(const kf_config& c);
+void launch_bilateral(kf_context* ctx, const uint16_t* raw, StageCopy stage = {});
 void launch_level_maps(kf_context* ctx, int level);
 // All three levels' maps after the bilateral pass in one launch (3 levels, a
 // camera size divisible by 4); false when it does not apply.
