@@ -607,6 +607,7 @@ kf_status kf_create(const kf_config* cfg, kf_context** out) {
       ctx->part_words = (static_cast<size_t>(bm.bx) * bm.by * bm.bz + 7) / 8;
       ctx->part_bits = dalloc<uint32_t>(ctx->part_words);
       ctx->free_bits = dalloc<uint32_t>(ctx->block_words);
+      ctx->class_bits = dalloc<uint32_t>(2 * ctx->block_words);
       KF_HIP(hipMemsetAsync(ctx->part_bits, 0, ctx->part_words * 4, ctx->stream));
       if (c.storage == KF_STORAGE_HASHED) {
         const BlockMap cm = coarse_map(ctx->grid);
@@ -681,6 +682,7 @@ void kf_destroy(kf_context* ctx) {
   dfree(ctx->skip_bits);
   dfree(ctx->part_bits);
   dfree(ctx->free_bits);
+  dfree(ctx->class_bits);
   dfree(ctx->coarse_bits);
   dfree(ctx->nbr_obs);
   dfree(ctx->counters);
@@ -1181,7 +1183,19 @@ extern "C" kf_status kf_block_bitmaps(kf_context* ctx, uint32_t* observed, uint3
       KF_HIP(hipMemcpyAsync(skip, ctx->skip_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (free_space != nullptr)
       KF_HIP(hipMemcpyAsync(free_space, ctx->free_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    // the raycaster's class words hold the same skip and free bits
+    std::vector<uint32_t> sk(ctx->block_words), fr(ctx->block_words), cls(2 * ctx->block_words);
+    KF_HIP(hipMemcpyAsync(sk.data(), ctx->skip_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipMemcpyAsync(fr.data(), ctx->free_bits, ctx->block_words * 4, hipMemcpyDeviceToHost, ctx->stream));
+    KF_HIP(hipMemcpyAsync(cls.data(), ctx->class_bits, ctx->block_words * 8, hipMemcpyDeviceToHost, ctx->stream));
     KF_HIP(hipStreamSynchronize(ctx->stream));
+    const BlockMap bm = block_map(ctx->grid);
+    const size_t nblocks = static_cast<size_t>(bm.bx) * bm.by * bm.bz;
+    for (size_t b = 0; b < nblocks; ++b) {
+      const uint32_t c = (cls[b >> 4] >> (2 * (b & 15))) & 3U;
+      const uint32_t want = ((sk[b >> 5] >> (b & 31)) & 1U) | (((fr[b >> 5] >> (b & 31)) & 1U) << 1);
+      if (c != want) fail(KF_ERR_DEVICE, "kf_block_bitmaps: the class words disagree with the skip and free maps");
+    }
   });
 }
 

@@ -630,9 +630,24 @@ __device__ inline void close_frame(IcpState* st, double* trajectory) {
 
 // free bit of a block = the block and its 26 neighbours (all inside the
 // grid) hold only fused voxels reading tsdf exactly 1.
+// Bit j of x at bit 2j (j < 16).
+__device__ inline uint32_t spread16(uint32_t x) {
+  x &= 0xFFFFU;
+  x = (x | (x << 8)) & 0x00FF00FFU;
+  x = (x | (x << 4)) & 0x0F0F0F0FU;
+  x = (x | (x << 2)) & 0x33333333U;
+  x = (x | (x << 1)) & 0x55555555U;
+  return x;
+}
+// The class words of 16 blocks from their skip and free bits.
+__device__ inline uint32_t class_word(uint32_t skip16, uint32_t free16) {
+  return spread16(skip16) | (spread16(free16) << 1);
+}
+
 __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restrict__ obs, uint32_t* __restrict__ skip,
                                                         const uint32_t* __restrict__ parts,
-                                                        uint32_t* __restrict__ free_bits, BlockMap bm, int nblocks,
+                                                        uint32_t* __restrict__ free_bits,
+                                                        uint32_t* __restrict__ class_bits, BlockMap bm, int nblocks,
                                                         IcpState* end_state, double* trajectory, CopyU16 keep) {
   keep.run();
   // end_state: this launch also closes the frame (one thread): the
@@ -652,13 +667,13 @@ __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restric
             break;
           }
   }
+  const unsigned long long m_free = __ballot(free_nb);
   {
-    const unsigned long long m = __ballot(free_nb);
     const int lane = threadIdx.x & 63;
     const int word0 = (blockIdx.x * 256 + (threadIdx.x & ~63)) >> 5;
     const int nwords = (nblocks + 31) >> 5;
-    if (lane == 0 && word0 < nwords) free_bits[word0] = static_cast<uint32_t>(m);
-    if (lane == 32 && word0 + 1 < nwords) free_bits[word0 + 1] = static_cast<uint32_t>(m >> 32);
+    if (lane == 0 && word0 < nwords) free_bits[word0] = static_cast<uint32_t>(m_free);
+    if (lane == 32 && word0 + 1 < nwords) free_bits[word0 + 1] = static_cast<uint32_t>(m_free >> 32);
   }
   if (b < nblocks) {
     const int x = b % bm.bx, y = (b / bm.bx) % bm.by, z = b / (bm.bx * bm.by);
@@ -680,6 +695,11 @@ __global__ void __launch_bounds__(256) skip_map_kernel(const uint32_t* __restric
   const int nwords = (nblocks + 31) >> 5;
   if (lane == 0 && word0 < nwords) skip[word0] = static_cast<uint32_t>(m);
   if (lane == 32 && word0 + 1 < nwords) skip[word0 + 1] = static_cast<uint32_t>(m >> 32);
+  // lanes 0, 16, 32, 48: the class word of their 16 blocks
+  const int cwords = (nblocks + 15) >> 4;
+  const int cw = 2 * word0 + (lane >> 4);
+  if ((lane & 15) == 0 && cw < cwords)
+    class_bits[cw] = class_word(static_cast<uint32_t>(m >> lane), static_cast<uint32_t>(m_free >> lane));
 }
 
 // Bit i of the result: block i of the 32 blocks whose part nibbles are the
@@ -708,7 +728,8 @@ __device__ inline uint32_t full_blocks(uint4 p) {
 __global__ void __launch_bounds__(64) skip_map_rows_kernel(const uint32_t* __restrict__ obs,
                                                              uint32_t* __restrict__ skip,
                                                              const uint32_t* __restrict__ parts,
-                                                             uint32_t* __restrict__ free_bits, BlockMap bm,
+                                                             uint32_t* __restrict__ free_bits,
+                                                             uint32_t* __restrict__ class_bits, BlockMap bm,
                                                              IcpState* end_state, double* trajectory, CopyU16 keep) {
   if (end_state != nullptr && blockIdx.x == 0 && threadIdx.x == 0) close_frame(end_state, trajectory);
   keep.run();
@@ -748,8 +769,12 @@ __global__ void __launch_bounds__(64) skip_map_rows_kernel(const uint32_t* __res
     full_lo &= (p_lo[k] >> 28) == 0xFU ? 1U : 0U;
     full_hi &= (p_hi[k] & 0xFU) == 0xFU ? 1U : 0U;
   }
-  skip[t] = ~(seen | (seen << 1) | (seen >> 1) | seen_lo | (seen_hi << 31));
-  free_bits[t] = interior ? full & ((full << 1) | full_lo) & ((full >> 1) | (full_hi << 31)) : 0U;
+  const uint32_t sk = ~(seen | (seen << 1) | (seen >> 1) | seen_lo | (seen_hi << 31));
+  const uint32_t fr = interior ? full & ((full << 1) | full_lo) & ((full >> 1) | (full_hi << 31)) : 0U;
+  skip[t] = sk;
+  free_bits[t] = fr;
+  class_bits[2 * t] = class_word(sk, fr);
+  class_bits[2 * t + 1] = class_word(sk >> 16, fr >> 16);
 }
 
 // coarse bit of a cell of 4^3 blocks = every block of it (inside the grid)
@@ -1287,11 +1312,13 @@ void launch_skip_map(kf_context* ctx, bool end_frame) {
   if (bm.bx % 32 == 0) {
     const int words = nblocks / 32;
     hipLaunchKernelGGL(skip_map_rows_kernel, dim3((words + 63) / 64), dim3(64), 0, ctx->stream, ctx->obs_bits,
-                       ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, end_frame ? ctx->icp : nullptr,
+                       ctx->skip_bits, ctx->part_bits, ctx->free_bits, ctx->class_bits, bm,
+                       end_frame ? ctx->icp : nullptr,
                        ctx->trajectory, keep);
   } else {
     hipLaunchKernelGGL(skip_map_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, ctx->obs_bits,
-                       ctx->skip_bits, ctx->part_bits, ctx->free_bits, bm, nblocks, end_frame ? ctx->icp : nullptr,
+                       ctx->skip_bits, ctx->part_bits, ctx->free_bits, ctx->class_bits, bm, nblocks,
+                       end_frame ? ctx->icp : nullptr,
                        ctx->trajectory, keep);
   }
   if (ctx->coarse_bits != nullptr) {

@@ -453,6 +453,9 @@ struct kf_context {
   // blocks without loads)
   uint32_t* part_bits = nullptr;
   uint32_t* free_bits = nullptr;
+  // both classes of block b in one word for the raycaster: bit 2 (b % 16) of
+  // word b / 16 is its skip bit, the bit above it its free bit
+  uint32_t* class_bits = nullptr;
   // hashed storage: one bit per cell of 4^3 blocks, set when every block of
   // the cell is a skip block (the raycaster steps over such a cell at once)
   uint32_t* coarse_bits = nullptr;

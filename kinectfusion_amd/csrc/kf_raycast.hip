@@ -17,8 +17,7 @@ template <int Kind, typename Store>
 __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, RayParams rp, Rigid pose_arg,
                                                        IcpState* state, float4* __restrict__ pts,
                                                        float4* __restrict__ nrm, uint32_t* __restrict__ col,
-                                                       const uint32_t* __restrict__ skip_bits,
-                                                       const uint32_t* __restrict__ free_bits,
+                                                       const uint32_t* __restrict__ class_bits,
                                                        const uint32_t* __restrict__ nbr_obs, BlockMap bm,
                                                        const uint32_t* __restrict__ coarse_bits, BlockMap cm,
                                                        int begin_level) {
@@ -42,7 +41,7 @@ __global__ void __launch_bounds__(256) raycast_kernel(Sampler<Kind, Store> vol, 
   int x, y;
   tile_pixel(tile, static_cast<int>(gridDim.x), static_cast<int>(threadIdx.x), x, y);
   if (x >= rp.w || y >= rp.h) return;
-  raycast_pixel(vol, rp, pose, x, y, pts, nrm, col, skip_bits, free_bits, nbr_obs, bm, coarse_bits, cm);
+  raycast_pixel(vol, rp, pose, x, y, pts, nrm, col, class_bits, nbr_obs, bm, coarse_bits, cm);
 }
 
 template <int Kind>
@@ -55,14 +54,14 @@ void launch_t(kf_context* ctx, const RayParams& rp, const Rigid& pose, IcpState*
         HashedStore<Kind>{hash_view(ctx), static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
         1.0F / ctx->grid.voxel};
     hipLaunchKernelGGL((raycast_kernel<Kind, HashedStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose,
-                       state, L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, ctx->nbr_obs, block_map(ctx->grid), ctx->coarse_bits, coarse_map(ctx->grid),
+                       state, L.model_pts, L.model_nrm, L.model_col, ctx->class_bits, ctx->nbr_obs, block_map(ctx->grid), ctx->coarse_bits, coarse_map(ctx->grid),
                        begin_level);
     return;
   }
   Sampler<Kind> vol{DenseStore<Kind>{static_cast<const Raw*>(ctx->voxels), ctx->colors}, ctx->grid,
                     1.0F / ctx->grid.voxel};
   hipLaunchKernelGGL((raycast_kernel<Kind, DenseStore<Kind>>), grid, dim3(256), 0, ctx->stream, vol, rp, pose, state,
-                     L.model_pts, L.model_nrm, L.model_col, ctx->skip_bits, ctx->free_bits, ctx->nbr_obs, block_map(ctx->grid), ctx->coarse_bits, coarse_map(ctx->grid),
+                     L.model_pts, L.model_nrm, L.model_col, ctx->class_bits, ctx->nbr_obs, block_map(ctx->grid), ctx->coarse_bits, coarse_map(ctx->grid),
                      begin_level);
 }
 
