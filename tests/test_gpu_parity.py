@@ -469,14 +469,17 @@ def test_small_workload_runs_and_is_deterministic(K, scene, torch_gpu):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
-def test_captured_frames_match_eager_frames(K, scene, torch_gpu):
+@pytest.mark.parametrize("radius", [3, 0])
+def test_captured_frames_match_eager_frames(K, scene, torch_gpu, radius):
     """Tracking frames replay one captured HIP graph; with stage timing on,
     every frame is enqueued eagerly instead.  Both give byte-identical poses,
     voxels and colours, also across an ICP-option change (which re-captures)
     and a frame without colour, with the frames enqueued back to back, and
-    the pyramid views after the last frame are that frame's in both."""
+    the pyramid views after the last frame are that frame's in both.  With
+    the bilateral filter on, its launch also stages the captured frame's
+    inputs; without it (radius 0) a copy launch does."""
     def run(stage_timing):
-        p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL, stage_timing=stage_timing))
+        p = K.Pipeline(pipeline_config(K, voxel=ROOM_VOXEL, radius=radius, stage_timing=stage_timing))
         loose = K.ProjectiveIcpOptions(max_point_distance=0.2)
         frames = []
         for i in range(9):
