@@ -400,7 +400,9 @@ kf_status kf_hashed_download(kf_context* ctx, uint32_t* keys, void* voxels, floa
  * observed: some voxel of the block is observed; skip: the block and its 26
  * neighbours are unobserved; free_space: the block and its 26 neighbours
  * hold only observed voxels reading tsdf exactly 1.  *words receives the
- * number of 32-bit words of each map; NULL maps are not copied. */
+ * number of 32-bit words of each map; NULL maps are not copied.  The call
+ * also checks the raycaster's per-block class words (skip and free bits
+ * side by side) against both maps: KF_ERR_DEVICE if they disagree. */
 kf_status kf_block_bitmaps(kf_context* ctx, uint32_t* observed, uint32_t* skip, uint32_t* free_space, int64_t* words);
 
 /* ---- MarchingCubes::extract (marching_cubes.cpp:478-507) on the device:
