@@ -20,6 +20,14 @@ def test_header_declares_entry_points():
         assert n in names
 
 
+def test_integration_guide_maps_every_entry_point():
+    """The integration guide's interface table names every declared entry
+    point, so a maintainer wiring the reference to the library finds each."""
+    guide = (ROOT / "INTEGRATION.md").read_text()
+    missing = [n for n in declared() if f"`{n}`" not in guide]
+    assert not missing, missing
+
+
 def test_library_exports_every_declared_symbol():
     from kinectfusion_amd import _native
 
